@@ -1,0 +1,202 @@
+/*
+ * psvwave.h -- C-ABI of the MI355X-native fdModel hot path
+ * (libpsvwave_amd.so).
+ *
+ * Drop-in boundary for psvWave's fdModel: every entry point below replaces a
+ * method of the reference's pybind11 class (src/psvWave.cpp:465-655, class
+ * fdModelExtended at :85-458) or of the C++ class it wraps
+ * (src/fdModel.h:35-318).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Every function returns PSV_OK (0) or an error status; the message of
+ *    the last error on the calling thread is psv_last_error().
+ *  - PSV_EVALUE marks the conditions where the reference throws
+ *    std::invalid_argument / std::length_error / py::value_error (surfaced
+ *    in Python as ValueError); PSV_ERUNTIME marks HIP failures.
+ *  - Host buffers are caller-owned and use the reference layout: grids
+ *    (nx, nz) with iz fastest, data (n_shots, nr, nt), snapshots
+ *    (n_shots, snapshots, nx, nz), vectors (free_parameters,).
+ *  - Calls are synchronous: results are complete when a call returns.  A
+ *    model is not re-entrant; distinct models are independent.
+ */
+#ifndef PSVWAVE_H
+#define PSVWAVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSV_OK 0
+#define PSV_EVALUE 1
+#define PSV_ERUNTIME 2
+
+typedef struct psv_model psv_model;
+
+/* Configuration with the 30 values of the reference's explicit-argument
+ * constructor (src/fdModel.h:50-64).  which_source_to_fire_in_which_shot is
+ * given in CSR form: shot k fires which_sources[which_offsets[k] ..
+ * which_offsets[k+1]). */
+typedef struct {
+  int nt, nx_inner, nz_inner, nx_inner_boundary, nz_inner_boundary;
+  double dx, dz, dt;
+  int np_boundary;
+  double np_factor;
+  double scalar_rho, scalar_vp, scalar_vs;
+  int npx, npz;
+  double peak_frequency, source_timeshift, delay_cycles_per_shot;
+  int n_sources, n_shots;
+  const int *ix_sources, *iz_sources;
+  const double *moment_angles;
+  const int *which_offsets, *which_sources;
+  int nr;
+  const int *ix_receivers, *iz_receivers;
+  int snapshot_interval;
+  const char *observed_data_folder, *stf_folder;
+} psv_params;
+
+/* Read-only attributes (src/psvWave.cpp:499-501, 557-588, 621-626). */
+typedef struct {
+  int nt, nx, nz, nx_inner, nz_inner, nx_inner_boundary, nz_inner_boundary;
+  int np_boundary, nx_free_parameters, nz_free_parameters, free_parameters;
+  int snapshot_interval, snapshots, n_shots, n_sources, nr, npx, npz;
+  double dx, dz, dt, np_factor, peak_frequency, source_timeshift,
+      delay_cycles_per_shot, scalar_rho, scalar_vp, scalar_vs, misfit;
+} psv_info;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *psv_last_error(void);
+
+/* Parsed and validated configuration, host only (no device work).  Covers
+ * fdModel::parse_configuration_file / parse_parameters / initialize_arrays
+ * (src/fdModel.cpp:390-523, 193-299) without allocating a model. */
+typedef struct psv_config psv_config;
+int psv_config_from_ini(const char *path, psv_config **out);
+int psv_config_from_params(const psv_params *p, psv_config **out);
+void psv_config_destroy(psv_config *c);
+int psv_config_get_info(const psv_config *c, psv_info *out);
+/* which_source_to_fire_in_which_shot (CSR), grid indices of sources and
+ * receivers including np_boundary; any pointer may be NULL. */
+int psv_config_get_lists(const psv_config *c, int *which_offsets,
+                         int *which_sources, int *ix_src, int *iz_src,
+                         int *ix_rec, int *iz_rec);
+/* stf (n_sources, nt), moment (n_sources, 2, 2), taper (nx, nz); any
+ * pointer may be NULL. */
+int psv_config_get_setup(const psv_config *c, double *stf, double *moment,
+                         double *taper);
+int psv_create_from_config(const psv_config *c, psv_model **out);
+
+/* fdModel(const char*) -- src/fdModel.cpp:16-27, psvWave.cpp:471. */
+int psv_create_from_ini(const char *path, psv_model **out);
+/* fdModel(nt, nx_inner, ...) -- src/fdModel.cpp:29-62. */
+int psv_create(const psv_params *p, psv_model **out);
+/* copy() -- src/psvWave.cpp:421-425, fdModel(const fdModel&) at
+ * src/fdModel.cpp:64-97.  Deep copy of all device state. */
+int psv_copy(const psv_model *m, psv_model **out);
+/* ~fdModel() -- src/fdModel.cpp:99-137. */
+void psv_destroy(psv_model *m);
+
+/* def_readonly attributes -- src/psvWave.cpp:499-588, 621-626. */
+int psv_get_info(const psv_model *m, psv_info *out);
+/* which_source_to_fire_in_which_shot (src/psvWave.cpp:586-588), CSR form:
+ * offsets has n_shots + 1 entries, sources n_sources. */
+int psv_get_which(const psv_model *m, int *offsets, int *sources);
+/* Source / receiver grid indices including the absorbing boundary
+ * (ix_sources etc. after src/fdModel.cpp:197-212); used by get_sources /
+ * get_receivers (src/psvWave.cpp:324-387). */
+int psv_get_source_indices(const psv_model *m, int *ix, int *iz);
+int psv_get_receiver_indices(const psv_model *m, int *ix, int *iz);
+/* Output folders from the [output] section, copied into buf (size len). */
+int psv_get_folders(const psv_model *m, char *observed, char *stf, int len);
+
+/* forward_simulate -- src/fdModel.cpp:526-787, psvWave.cpp:112-158, 476-498.
+ * omp_threads_override of the reference has no device meaning and is not
+ * part of this call.  output_wavefields writes snapshots/vxNNNNN.txt and
+ * vzNNNNN.txt every 10 steps as the reference does. */
+int psv_forward(psv_model *m, int i_shot, int store_fields, int verbose,
+                int output_wavefields);
+/* adjoint_simulate -- src/fdModel.cpp:789-969, psvWave.cpp:160-205. */
+int psv_adjoint(psv_model *m, int i_shot, int verbose);
+/* calculate_l2_misfit -- src/fdModel.cpp:1128-1145.  Also stored as the
+ * misfit attribute. */
+int psv_calculate_l2_misfit(psv_model *m, double *misfit);
+/* calculate_l2_adjoint_sources -- src/fdModel.cpp:1147-1162. */
+int psv_calculate_l2_adjoint_sources(psv_model *m);
+/* reset_kernels -- src/fdModel.cpp:1310-1324. */
+int psv_reset_kernels(psv_model *m);
+/* map_kernels_to_velocity -- src/fdModel.cpp:1164-1183. */
+int psv_map_kernels_to_velocity(psv_model *m);
+/* run_model -- src/fdModel.cpp:1291-1308 (C++-only in the reference). */
+int psv_run_model(psv_model *m, int verbose, int simulate_adjoint);
+
+/* get_parameter_fields / set_parameter_fields -- src/psvWave.cpp:248-297;
+ * set also runs update_from_velocity (src/fdModel.cpp:1034-1051). */
+int psv_get_parameter_fields(const psv_model *m, double *vp, double *vs,
+                             double *rho);
+int psv_set_parameter_fields(psv_model *m, const double *vp, const double *vs,
+                             const double *rho);
+/* get_kernels -- src/psvWave.cpp:299-306 (vp, vs, rho velocity kernels). */
+int psv_get_kernels(const psv_model *m, double *vp_k, double *vs_k,
+                    double *rho_k);
+/* The Lame-basis kernels lambda_kernel, mu_kernel, density_l_kernel
+ * (src/fdModel.h:224-226); public fields of the reference C++ class. */
+int psv_get_lame_kernels(const psv_model *m, double *lambda_k, double *mu_k,
+                         double *rho_l_k);
+/* get/set_synthetic_data, get/set_observed_data --
+ * src/psvWave.cpp:308-322, 389-419, 427-457. */
+int psv_get_synthetic_data(const psv_model *m, double *ux, double *uz);
+int psv_set_synthetic_data(psv_model *m, const double *ux, const double *uz);
+int psv_get_observed_data(const psv_model *m, double *ux, double *uz);
+int psv_set_observed_data(psv_model *m, const double *ux, const double *uz);
+/* a_stf_ux / a_stf_uz (src/fdModel.h:246-247). */
+int psv_get_adjoint_sources(const psv_model *m, double *ux, double *uz);
+/* get_snapshots -- src/psvWave.cpp:90-97: five arrays of
+ * (n_shots, snapshots, nx, nz). */
+int psv_get_snapshots(const psv_model *m, double *vx, double *vz, double *txx,
+                      double *tzz, double *txz);
+/* One (shot, snapshot) slice of get_snapshots, five (nx, nz) arrays. */
+int psv_get_snapshot(const psv_model *m, int i_shot, int i_snapshot,
+                     double *vx, double *vz, double *txx, double *tzz,
+                     double *txz);
+/* get_model_vector / set_model_vector / get_gradient_vector --
+ * src/fdModel.cpp:1358-1515.  n must equal free_parameters. */
+int psv_get_model_vector(const psv_model *m, double *v, int64_t n);
+int psv_set_model_vector(psv_model *m, const double *v, int64_t n);
+int psv_get_gradient_vector(const psv_model *m, double *v, int64_t n);
+/* load_vector -- src/fdModel.cpp:1517-1572 (reads 3*nxf*nzf values). */
+int psv_load_vector(const psv_model *m, const char *path, int verbose,
+                    double *v, int64_t n);
+/* Setup products (src/fdModel.cpp:214-246, 263-298): stf (n_sources, nt),
+ * moment (n_sources, 2, 2), taper (nx, nz). */
+int psv_get_setup(const psv_model *m, double *stf, double *moment,
+                  double *taper);
+
+/* Text formats of write_receivers / write_sources / write_kernels
+ * (src/fdModel.cpp:971-1032, 1326-1356) and load_receivers / load_model
+ * (src/fdModel.cpp:1053-1126, 1185-1289). */
+int psv_write_receivers(const psv_model *m, const char *prefix);
+int psv_write_sources(const psv_model *m);
+int psv_write_kernels(const psv_model *m);
+int psv_load_receivers(psv_model *m, int verbose);
+int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
+                   const char *vs_path, int verbose);
+
+/* Tuning and measurement (no reference counterpart).  variant: 0 fused
+ * step kernel with 64x8 tiles, 1 fused 64x16, 2 split (two launches per
+ * step).  xcd_remap: 1 keeps neighbouring tiles on one XCD.
+ * timing_stride k > 0 brackets every k-th step launch with HIP events on
+ * the model's stream; psv_get_step_timing returns the summed milliseconds
+ * and the number of bracketed launches since the last reset. */
+int psv_set_step_variant(psv_model *m, int variant, int xcd_remap);
+int psv_set_timing(psv_model *m, int timing_stride);
+int psv_get_step_timing(psv_model *m, double *total_ms, int64_t *timed,
+                        int64_t *launches);
+int psv_reset_step_timing(psv_model *m);
+/* Device bytes held by the model. */
+int psv_device_bytes(const psv_model *m, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

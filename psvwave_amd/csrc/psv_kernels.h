@@ -1,0 +1,179 @@
+// psv_kernels.h -- device data layout, per-point stencil arithmetic and the
+// launch interface of the MI355X fdModel kernels.
+//
+// Grids on the device are x-fastest ("[iz][ix]"): wavefront lanes run along
+// x, so every row segment a wave touches is one contiguous, coalesced
+// transaction.  The stencil arrays (state and materials) carry a zero margin
+// of kHaloX columns / kHaloZ rows around the tiled extent so the LDS halo
+// loads need no bounds checks; all other grids (parameters, kernels,
+// snapshots) are compact, index iz*nx + ix.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace psv {
+
+constexpr int kTileX = 64;   // tile width = one wavefront of lanes
+constexpr int kHaloX = 4;    // margin columns left of x = 0
+constexpr int kHaloZ = 4;    // margin rows above z = 0
+constexpr int kBlock = 256;  // threads per block for the step kernels
+
+// 4th-order staggered coefficients (src/fdModel.h:201-202).
+constexpr double kC1 = 9.0 / 8.0;
+constexpr double kC2 = 1.0 / 24.0;
+
+// One injection term applied by the owner of a grid point after its
+// velocity update.  Forward: a moment-tensor term (src/fdModel.cpp:681-751),
+// value ((coef * stf[src][it]) * dt) * b / D4 with coef = +-M or +-0.25 M.
+// Adjoint: an adjoint source (src/fdModel.cpp:951-958), value
+// ((dt * b) * a[src][it]) / (dx * dz) with src = receiver index.
+struct InjEntry {
+  int local;  // (z - z0) * kTileX + (x - x0) inside the owning tile
+  int comp;   // 0 = vx, 1 = vz
+  int src;    // source (forward) or receiver (adjoint) index
+  int pad;
+  double coef;
+};
+
+// Arguments of one time step of one shot.
+struct StepArgs {
+  int nx, nz, pitch, ntx, ntz, tz;  // grid, row pitch, tiles, tile height
+  long long origin;                  // padded index of (0, 0)
+  const double *vx_i, *vz_i, *txx_i, *tzz_i, *txz_i;  // state at step start
+  double *vx_o, *vz_o, *txx_o, *tzz_o, *txz_o;        // state after the step
+  const double *lm, *mu, *b;                          // padded materials
+  const double *taper_lut;                            // np_boundary + 1 values
+  int np;
+  double dts;  // +dt forward, -dt adjoint (stencil sign)
+  double dt, dx, dz, dxdz, d4x, d4z;
+  int it, nt, adjoint;
+  // snapshot slot of this step (compact [iz][ix]); written forward, read
+  // adjoint when do_snap is set
+  int do_snap;
+  double *snap[5];  // vx, vz, txx, tzz, txz
+  // kernel correlation (adjoint, do_snap steps): Lame kernels, compact
+  double *lam_k, *mu_k, *rho_k;
+  int kx0, kx1, kz0, kz1;
+  double wcorr;  // snapshot_interval * dt
+  // injection lists for this shot, CSR by tile
+  const int *tile_off;
+  const InjEntry *entries;
+  const double *stf;             // [n_sources][nt] (forward)
+  const double *adj_x, *adj_z;   // this shot's adjoint sources [nr][nt]
+  // receiver record (forward): this shot's traces [nr][nt]
+  int nr;
+  const long long *rec_idx;      // padded index of each receiver
+  double *rtf_x, *rtf_z;
+  int xcd_remap;                 // map consecutive tiles to one XCD's L2
+  int ntiles;
+};
+
+// Pointwise arithmetic, written once for the device kernels.  Expression
+// trees follow the reference operation by operation; with contraction off
+// the results are bitwise those of the reference's double build.
+//
+// Stress at one point from velocities v*, accessed at offsets sx (one column)
+// and sz (one row).  src/fdModel.cpp:615-642 (forward), :882-909 (adjoint).
+__host__ __device__ inline void stress_point(const double *vx,
+                                             const double *vz, long long i,
+                                             long long sx, long long sz,
+                                             double lm, double mu,
+                                             double taper, double dts,
+                                             double dx, double dz, double &txx,
+                                             double &tzz, double &txz) {
+  double dvx_x = kC1 * (vx[i + sx] - vx[i]) + kC2 * (vx[i - sx] - vx[i + 2 * sx]);
+  double dvz_z = kC1 * (vz[i] - vz[i - sz]) + kC2 * (vz[i - 2 * sz] - vz[i + sz]);
+  double dvx_z = kC1 * (vx[i + sz] - vx[i]) + kC2 * (vx[i - sz] - vx[i + 2 * sz]);
+  double dvz_x = kC1 * (vz[i] - vz[i - sx]) + kC2 * (vz[i - 2 * sx] - vz[i + sx]);
+  double la = lm - 2 * mu;  // la array of src/fdModel.cpp:1046, recomputed
+  txx = taper * (txx + dts * (lm * dvx_x / dx + la * dvz_z / dz));
+  tzz = taper * (tzz + dts * (la * dvx_x / dx + lm * dvz_z / dz));
+  txz = taper * (txz + dts * mu * (dvx_z / dz + dvz_x / dx));
+}
+
+// Velocity at one point from stresses.  src/fdModel.cpp:661-676 (forward),
+// :931-946 (adjoint); b_vx == b_vz == 1/rho (src/fdModel.cpp:1047-1048).
+__host__ __device__ inline void velocity_point(
+    const double *txx, const double *tzz, const double *txz, long long i,
+    long long sx, long long sz, double b, double taper, double dts, double dx,
+    double dz, double &vx, double &vz) {
+  double ax = (kC1 * (txx[i] - txx[i - sx]) + kC2 * (txx[i - 2 * sx] - txx[i + sx])) / dx +
+              (kC1 * (txz[i] - txz[i - sz]) + kC2 * (txz[i - 2 * sz] - txz[i + sz])) / dz;
+  double az = (kC1 * (txz[i + sx] - txz[i]) + kC2 * (txz[i - sx] - txz[i + 2 * sx])) / dx +
+              (kC1 * (tzz[i + sz] - tzz[i]) + kC2 * (tzz[i - sz] - tzz[i + 2 * sz])) / dz;
+  vx = taper * (vx + b * dts * ax);
+  vz = taper * (vz + b * dts * az);
+}
+
+// Correlation of the adjoint state with a forward snapshot at one point of
+// the kernel region.  src/fdModel.cpp:832-858.
+__host__ __device__ inline void correlate_point(
+    double w, double lm, double mu, double vx, double vz, double txx,
+    double tzz, double txz, double avx, double avz, double atxx, double atzz,
+    double atxz, double &rho_k, double &lam_k, double &mu_k) {
+  double la = lm - 2 * mu;
+  rho_k -= w * (avx * vx + avz * vz);
+  double qa1 = atxx - (atzz * la) / lm;
+  double qa2 = atzz - (atxx * la) / lm;
+  double q1 = txx - (tzz * la) / lm;
+  double q2 = tzz - (txx * la) / lm;
+  double d = lm - ((la * la) / (lm));
+  lam_k += w * ((qa1 + qa2) * (q1 + q2)) / (d * d);
+  mu_k += w * 2 * ((q1 * qa1 + q2 * qa2) / (d * d) + 2 * (txz * atxz / (4 * mu * mu)));
+}
+
+// Taper of a point from its nesting count (src/fdModel.cpp:263-298):
+// count = min(np, 1 + distance to the nearest grid edge).
+__host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np) {
+  int d = x < nx - 1 - x ? x : nx - 1 - x;
+  int dz = z < nz - 1 - z ? z : nz - 1 - z;
+  d = d < dz ? d : dz;
+  return d + 1 < np ? d + 1 : np;
+}
+
+// Step-kernel variants.
+enum StepVariant { kFusedTz8 = 0, kFusedTz16 = 1, kSplitTz8 = 2 };
+
+// Host launchers (psv_kernels.hip).  All enqueue on `stream`.
+hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream);
+int step_tile_height(int variant);
+bool step_is_inplace(int variant);
+
+hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
+hipError_t launch_update_from_velocity(const double *vp, const double *vs,
+                                       const double *rho, double *lm,
+                                       double *mu, double *b, int nx, int nz,
+                                       int pitch, long long origin,
+                                       hipStream_t s);
+hipError_t launch_adjoint_sources(const double *sx, const double *sz,
+                                  const double *ox, const double *oz,
+                                  double *ax, double *az, long long n,
+                                  hipStream_t s);
+hipError_t launch_map_kernels(const double *vp, const double *vs,
+                              const double *rho, const double *lam_k,
+                              const double *mu_k, const double *rho_l,
+                              double *vp_k, double *vs_k, double *rho_v,
+                              long long n, hipStream_t s);
+// misfit: partials must hold 2 * kMisfitBlocks doubles, out 1 double.
+constexpr int kMisfitBlocks = 512;
+hipError_t launch_misfit(const double *sx, const double *sz, const double *ox,
+                         const double *oz, long long n, double half_dt,
+                         double *partials, double *out, hipStream_t s);
+hipError_t launch_model_vector(int mode, double *vp, double *vs, double *rho,
+                               double *vec, int nx, int nxf, int nzf, int npx,
+                               int npz, int gx0, int gz0, hipStream_t s);
+hipError_t launch_gradient_vector(const double *vp_k, const double *vs_k,
+                                  const double *rho_k, double *vec, int nx,
+                                  int nxf, int nzf, int npx, int npz, int gx0,
+                                  int gz0, hipStream_t s);
+// Batched layout change between compact [slice][iz][ix] (device) and the
+// reference's [slice][ix][iz].  to_ref = 1: device -> reference.
+hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
+                            int slices, int to_ref, hipStream_t s);
+// Padded stencil array -> compact [iz][ix].
+hipError_t launch_unpad(const double *in, double *out, int nx, int nz,
+                        int pitch, long long origin, hipStream_t s);
+
+}  // namespace psv

@@ -1,0 +1,575 @@
+// psv_kernels.hip -- hand-written gfx950 kernels for the fdModel hot path.
+//
+// The time step (stress then velocity, src/fdModel.cpp:599-678 forward,
+// :864-948 adjoint) is one launch per step in the fused variant:
+//
+//   phase A  stage vx, vz of the tile plus a 4-point halo in LDS;
+//   phase B  update the three stresses on the tile plus a 2-point ring
+//            (the ring is recomputed redundantly by neighbouring tiles), keep
+//            them in LDS and store the tile's own stresses;
+//   phase C  update vx, vz on the tile from the LDS stresses, apply this
+//            tile's injection terms in reference order, store.
+//
+// A step reads one buffer set and writes the other (ping-pong), so no tile
+// ever reads what another tile of the same launch writes.  Snapshot stores
+// (forward), kernel correlation (adjoint) and the receiver record (forward)
+// ride in the same launch on the owning threads.
+//
+// The split variant does the same work as two in-place launches per step
+// (stress, velocity) without redundant compute; it is kept as an
+// independent cross-check of the fused kernel and as a tuning alternative.
+#include "psv_kernels.h"
+
+namespace psv {
+
+namespace {
+
+__device__ inline long long pidx(const StepArgs &a, int x, int z) {
+  return a.origin + (long long)z * a.pitch + x;
+}
+
+__device__ inline double taper_at(const StepArgs &a, int x, int z) {
+  int c = taper_count(x, z, a.nx, a.nz, a.np);
+  return c >= a.np ? 1.0 : a.taper_lut[c];
+}
+
+__device__ inline bool interior(const StepArgs &a, int x, int z) {
+  return x >= 2 && x < a.nx - 2 && z >= 2 && z < a.nz - 2;
+}
+
+// Tile handled by this block.  With xcd_remap, blocks that the dispatcher
+// sends to the same XCD (block id mod 8) get a contiguous run of tiles, so
+// halo re-reads between neighbouring tiles hit that XCD's L2.
+__device__ inline int block_tile(const StepArgs &a) {
+  int b = blockIdx.x;
+  if (!a.xcd_remap) return b;
+  int chunk = (a.ntiles + 7) / 8;
+  return (b % 8) * chunk + b / 8;
+}
+
+// Receiver record at the start of step it (src/fdModel.cpp:576-596):
+// running sum of dt * v / (dx * dz).
+__device__ inline void record_receivers(const StepArgs &a) {
+  for (int r = threadIdx.x; r < a.nr; r += blockDim.x) {
+    long long li = a.rec_idx[r];
+    double incx = a.dt * a.vx_i[li] / a.dxdz;
+    double incz = a.dt * a.vz_i[li] / a.dxdz;
+    long long k = (long long)r * a.nt + a.it;
+    if (a.it == 0) {
+      a.rtf_x[k] = incx;
+      a.rtf_z[k] = incz;
+    } else {
+      a.rtf_x[k] = a.rtf_x[k - 1] + incx;
+      a.rtf_z[k] = a.rtf_z[k - 1] + incz;
+    }
+  }
+}
+
+// Owner-side application of this tile's injection terms, in list order.
+__device__ inline void apply_injection(const StepArgs &a, int tile, int local,
+                                       double b, double &vx, double &vz) {
+  int e0 = a.tile_off[tile], e1 = a.tile_off[tile + 1];
+  for (int e = e0; e < e1; ++e) {
+    const InjEntry &E = a.entries[e];
+    if (E.local != local) continue;
+    double term;
+    if (!a.adjoint) {
+      double S = a.stf[(long long)E.src * a.nt + a.it];
+      term = E.coef * S * a.dt * b / (E.comp == 0 ? a.d4x : a.d4z);
+    } else {
+      const double *src = E.comp == 0 ? a.adj_x : a.adj_z;
+      term = a.dt * b * src[(long long)E.src * a.nt + a.it] / a.dxdz;
+    }
+    if (E.comp == 0)
+      vx += term;
+    else
+      vz += term;
+  }
+}
+
+// Everything the owner of (x, z) does with the state at the start of the
+// step: snapshot store (forward, src/fdModel.cpp:555-572) or correlation
+// (adjoint, src/fdModel.cpp:817-861).
+__device__ inline void owner_snapshot(const StepArgs &a, int x, int z,
+                                      double vx, double vz, double txx,
+                                      double tzz, double txz) {
+  long long ci = (long long)z * a.nx + x;
+  if (!a.adjoint) {
+    a.snap[0][ci] = vx;
+    a.snap[1][ci] = vz;
+    a.snap[2][ci] = txx;
+    a.snap[3][ci] = tzz;
+    a.snap[4][ci] = txz;
+  } else if (x >= a.kx0 && x < a.kx1 && z >= a.kz0 && z < a.kz1) {
+    long long pi = pidx(a, x, z);
+    double rk = a.rho_k[ci], lk = a.lam_k[ci], mk = a.mu_k[ci];
+    correlate_point(a.wcorr, a.lm[pi], a.mu[pi], vx, vz, txx, tzz, txz,
+                    a.snap[0][ci], a.snap[1][ci], a.snap[2][ci], a.snap[3][ci],
+                    a.snap[4][ci], rk, lk, mk);
+    a.rho_k[ci] = rk;
+    a.lam_k[ci] = lk;
+    a.mu_k[ci] = mk;
+  }
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
+  constexpr int WV = kTileX + 8, HV = TZ + 8;  // velocity halo region
+  constexpr int WS = kTileX + 4, HS = TZ + 4;  // stress ring region
+  __shared__ double s_vx[HV * WV], s_vz[HV * WV];
+  __shared__ double s_txx[HS * WS], s_tzz[HS * WS], s_txz[HS * WS];
+
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  const int tid = threadIdx.x;
+
+  // phase A: velocities of the tile with a 4-point halo (always inside the
+  // zero margin of the padded arrays).
+  for (int i = tid; i < HV * WV; i += kBlock) {
+    int r = i / WV, c = i - r * WV;
+    long long g = pidx(a, x0 - 4 + c, z0 - 4 + r);
+    s_vx[i] = a.vx_i[g];
+    s_vz[i] = a.vz_i[g];
+  }
+  if (!a.adjoint && tile == 0) record_receivers(a);
+  __syncthreads();
+
+  // phase B: stresses on the tile plus a 2-point ring.
+  for (int i = tid; i < HS * WS; i += kBlock) {
+    int r = i / WS, c = i - r * WS;
+    int x = x0 - 2 + c, z = z0 - 2 + r;
+    double txx = 0.0, tzz = 0.0, txz = 0.0;
+    if (x >= 0 && x < a.nx && z >= 0 && z < a.nz) {
+      long long g = pidx(a, x, z);
+      txx = a.txx_i[g];
+      tzz = a.tzz_i[g];
+      txz = a.txz_i[g];
+      bool own = r >= 2 && r < TZ + 2 && c >= 2 && c < kTileX + 2;
+      if (own && a.do_snap && !a.adjoint) {
+        long long ci = (long long)z * a.nx + x;
+        a.snap[2][ci] = txx;
+        a.snap[3][ci] = tzz;
+        a.snap[4][ci] = txz;
+      }
+      if (interior(a, x, z)) {
+        stress_point(s_vx, s_vz, (long long)(r + 2) * WV + (c + 2), 1, WV,
+                     a.lm[g], a.mu[g], taper_at(a, x, z), a.dts, a.dx, a.dz,
+                     txx, tzz, txz);
+      }
+      if (own) {
+        a.txx_o[g] = txx;
+        a.tzz_o[g] = tzz;
+        a.txz_o[g] = txz;
+      }
+    }
+    s_txx[i] = txx;
+    s_tzz[i] = tzz;
+    s_txz[i] = txz;
+  }
+  __syncthreads();
+
+  // phase C: velocities on the tile.
+  for (int i = tid; i < TZ * kTileX; i += kBlock) {
+    int r = i / kTileX, c = i - r * kTileX;
+    int x = x0 + c, z = z0 + r;
+    if (x >= a.nx || z >= a.nz) continue;
+    int lv = (r + 4) * WV + (c + 4);
+    double vx = s_vx[lv], vz = s_vz[lv];
+    long long g = pidx(a, x, z);
+    if (a.do_snap) {
+      if (!a.adjoint) {
+        long long ci = (long long)z * a.nx + x;
+        a.snap[0][ci] = vx;
+        a.snap[1][ci] = vz;
+      } else {
+        owner_snapshot(a, x, z, vx, vz, a.txx_i[g], a.tzz_i[g], a.txz_i[g]);
+      }
+    }
+    double b = a.b[g];
+    if (interior(a, x, z)) {
+      velocity_point(s_txx, s_tzz, s_txz, (long long)(r + 2) * WS + (c + 2), 1,
+                     WS, b, taper_at(a, x, z), a.dts, a.dx, a.dz, vx, vz);
+    }
+    apply_injection(a, tile, r * kTileX + c, b, vx, vz);
+    a.vx_o[g] = vx;
+    a.vz_o[g] = vz;
+  }
+}
+
+// Split variant, launch 1 of 2: start-of-step owner work plus the in-place
+// stress update.  Velocities are read-only here.
+template <int TZ>
+__global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  if (!a.adjoint && tile == 0) record_receivers(a);
+  for (int i = threadIdx.x; i < TZ * kTileX; i += kBlock) {
+    int r = i / kTileX, c = i - r * kTileX;
+    int x = x0 + c, z = z0 + r;
+    if (x >= a.nx || z >= a.nz) continue;
+    long long g = pidx(a, x, z);
+    double txx = a.txx_i[g], tzz = a.tzz_i[g], txz = a.txz_i[g];
+    if (a.do_snap)
+      owner_snapshot(a, x, z, a.vx_i[g], a.vz_i[g], txx, tzz, txz);
+    if (interior(a, x, z)) {
+      stress_point(a.vx_i, a.vz_i, g, 1, a.pitch, a.lm[g], a.mu[g],
+                   taper_at(a, x, z), a.dts, a.dx, a.dz, txx, tzz, txz);
+      a.txx_o[g] = txx;
+      a.tzz_o[g] = tzz;
+      a.txz_o[g] = txz;
+    }
+  }
+}
+
+// Split variant, launch 2 of 2: in-place velocity update and injection.
+template <int TZ>
+__global__ void __launch_bounds__(kBlock) step_split_velocity(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  for (int i = threadIdx.x; i < TZ * kTileX; i += kBlock) {
+    int r = i / kTileX, c = i - r * kTileX;
+    int x = x0 + c, z = z0 + r;
+    if (x >= a.nx || z >= a.nz) continue;
+    long long g = pidx(a, x, z);
+    double vx = a.vx_i[g], vz = a.vz_i[g], b = a.b[g];
+    if (interior(a, x, z)) {
+      velocity_point(a.txx_o, a.tzz_o, a.txz_o, g, 1, a.pitch, b,
+                     taper_at(a, x, z), a.dts, a.dx, a.dz, vx, vz);
+    }
+    apply_injection(a, tile, r * kTileX + c, b, vx, vz);
+    a.vx_o[g] = vx;
+    a.vz_o[g] = vz;
+  }
+}
+
+__global__ void fill_kernel(double *p, long long n, double v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// src/fdModel.cpp:1034-1051, compact parameters -> padded materials.
+__global__ void update_from_velocity_kernel(const double *vp, const double *vs,
+                                            const double *rho, double *lm,
+                                            double *mu, double *b, int nx,
+                                            int nz, int pitch,
+                                            long long origin) {
+  long long n = (long long)nx * nz;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int z = (int)(i / nx), x = (int)(i - (long long)z * nx);
+  long long g = origin + (long long)z * pitch + x;
+  double r = rho[i];
+  mu[g] = (vs[i] * vs[i]) * r;
+  lm[g] = (vp[i] * vp[i]) * r;
+  b[g] = 1.0 / r;
+}
+
+// src/fdModel.cpp:1147-1162
+__global__ void adjoint_sources_kernel(const double *sx, const double *sz,
+                                       const double *ox, const double *oz,
+                                       double *ax, double *az, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ax[i] = sx[i] - ox[i];
+  az[i] = sz[i] - oz[i];
+}
+
+// src/fdModel.cpp:1164-1183 (b_vx = 1/rho recomputed bitwise)
+__global__ void map_kernels_kernel(const double *vp, const double *vs,
+                                   const double *rho, const double *lam_k,
+                                   const double *mu_k, const double *rho_l,
+                                   double *vp_k, double *vs_k, double *rho_v,
+                                   long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p = vp[i], s = vs[i], lk = lam_k[i], mk = mu_k[i];
+  double b = 1.0 / rho[i];
+  vp_k[i] = 2 * p * lk / b;
+  vs_k[i] = (2 * s * mk - 4 * s * lk) / b;
+  rho_v[i] = rho_l[i] + (p * p - 2 * s * s) * lk + s * s * mk;
+}
+
+// Compensated (TwoSum) accumulation: the misfit is the sum of millions of
+// positive terms, and a fixed-order compensated tree keeps the result within
+// a few ulp of the exact sum, independent of launch shape.
+struct Acc {
+  double s, c;
+};
+
+__device__ inline void acc_add(Acc &a, double x) {
+  double t = a.s + x;
+  double bb = t - a.s;
+  a.c += (a.s - (t - bb)) + (x - bb);
+  a.s = t;
+}
+
+__device__ inline Acc acc_merge(Acc a, Acc b) {
+  Acc r{a.s, a.c + b.c};
+  acc_add(r, b.s);
+  return r;
+}
+
+__device__ inline Acc block_reduce(Acc v) {
+  __shared__ double ss[kBlock], sc[kBlock];
+  ss[threadIdx.x] = v.s;
+  sc[threadIdx.x] = v.c;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      Acc o = acc_merge(Acc{ss[threadIdx.x], sc[threadIdx.x]},
+                        Acc{ss[threadIdx.x + h], sc[threadIdx.x + h]});
+      ss[threadIdx.x] = o.s;
+      sc[threadIdx.x] = o.c;
+    }
+    __syncthreads();
+  }
+  return Acc{ss[0], sc[0]};
+}
+
+// Terms of src/fdModel.cpp:1140-1141: (0.5 * dt) * (obs - syn)^2.
+__global__ void __launch_bounds__(kBlock)
+    misfit_partial_kernel(const double *sx, const double *sz, const double *ox,
+                          const double *oz, long long n, double half_dt,
+                          double *partials) {
+  Acc a{0.0, 0.0};
+  long long stride = (long long)gridDim.x * kBlock;
+  for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += stride) {
+    double d1 = ox[i] - sx[i];
+    double d2 = oz[i] - sz[i];
+    acc_add(a, half_dt * (d1 * d1));
+    acc_add(a, half_dt * (d2 * d2));
+  }
+  Acc r = block_reduce(a);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = r.s;
+    partials[2 * blockIdx.x + 1] = r.c;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    misfit_final_kernel(const double *partials, int nparts, double *out) {
+  Acc a{0.0, 0.0};
+  for (int i = threadIdx.x; i < nparts; i += kBlock)
+    a = acc_merge(a, Acc{partials[2 * i], partials[2 * i + 1]});
+  Acc r = block_reduce(a);
+  if (threadIdx.x == 0) out[0] = r.s + r.c;
+}
+
+// Model vector <-> parameter fields (src/fdModel.cpp:1358-1457).  One thread
+// per parameter cell; the parameter index is x-fastest.
+__global__ void model_vector_kernel(int mode, double *vp, double *vs,
+                                    double *rho, double *vec, int nx, int nxf,
+                                    int nzf, int npx, int npz, int gx0,
+                                    int gz0) {
+  int ncx = nxf / npx, ncz = nzf / npz;
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  int npp = ncx * ncz;
+  if (ip >= npp) return;
+  int iz = ip / ncx, ix = ip - iz * ncx;
+  int gix = ix * npx + gx0, giz = iz * npz + gz0;
+  if (mode == 0) {  // get: cell average
+    double a = 0, b = 0, c = 0;
+    for (int sx = 0; sx < npx; ++sx)
+      for (int sz = 0; sz < npz; ++sz) {
+        long long i = (long long)(giz + sz) * nx + (gix + sx);
+        a += vp[i] / (npx * npz);
+        b += vs[i] / (npx * npz);
+        c += rho[i] / (npx * npz);
+      }
+    vec[ip] = a;
+    vec[ip + npp] = b;
+    vec[ip + 2 * npp] = c;
+  } else {  // set: fill the cell
+    for (int sx = 0; sx < npx; ++sx)
+      for (int sz = 0; sz < npz; ++sz) {
+        long long i = (long long)(giz + sz) * nx + (gix + sx);
+        vp[i] = vec[ip];
+        vs[i] = vec[ip + npp];
+        rho[i] = vec[ip + 2 * npp];
+      }
+  }
+}
+
+// src/fdModel.cpp:1459-1515: cell sum of the velocity kernels.
+__global__ void gradient_vector_kernel(const double *vp_k, const double *vs_k,
+                                       const double *rho_k, double *vec,
+                                       int nx, int nxf, int nzf, int npx,
+                                       int npz, int gx0, int gz0) {
+  int ncx = nxf / npx, ncz = nzf / npz;
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  int npp = ncx * ncz;
+  if (ip >= npp) return;
+  int iz = ip / ncx, ix = ip - iz * ncx;
+  int gix = ix * npx + gx0, giz = iz * npz + gz0;
+  double a = 0, b = 0, c = 0;
+  for (int sx = 0; sx < npx; ++sx)
+    for (int sz = 0; sz < npz; ++sz) {
+      long long i = (long long)(giz + sz) * nx + (gix + sx);
+      a += vp_k[i];
+      b += vs_k[i];
+      c += rho_k[i];
+    }
+  vec[ip] = a;
+  vec[ip + npp] = b;
+  vec[ip + 2 * npp] = c;
+}
+
+// 32x32 LDS tile transpose between [slice][iz][ix] and [slice][ix][iz].
+__global__ void transpose_kernel(const double *in, double *out, int nx, int nz,
+                                 int to_ref) {
+  __shared__ double t[32][33];
+  long long slice = (long long)blockIdx.z * nx * nz;
+  in += slice;
+  out += slice;
+  // rows/cols of the input matrix
+  int rows = to_ref ? nz : nx, cols = to_ref ? nx : nz;
+  int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    int r = r0 + k, c = c0 + threadIdx.x;
+    if (r < rows && c < cols) t[k][threadIdx.x] = in[(long long)r * cols + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    int c = c0 + k, r = r0 + threadIdx.x;
+    if (r < rows && c < cols) out[(long long)c * rows + r] = t[threadIdx.x][k];
+  }
+}
+
+__global__ void unpad_kernel(const double *in, double *out, int nx, int nz,
+                             int pitch, long long origin) {
+  long long n = (long long)nx * nz;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int z = (int)(i / nx), x = (int)(i - (long long)z * nx);
+  out[i] = in[origin + (long long)z * pitch + x];
+}
+
+inline unsigned blocks_for(long long n, int bs) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+}  // namespace
+
+int step_tile_height(int variant) { return variant == kFusedTz16 ? 16 : 8; }
+bool step_is_inplace(int variant) { return variant == kSplitTz8; }
+
+hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s) {
+  int nblocks = a.xcd_remap ? 8 * ((a.ntiles + 7) / 8) : a.ntiles;
+  switch (variant) {
+    case kFusedTz8:
+      hipLaunchKernelGGL(step_fused<8>, dim3(nblocks), dim3(kBlock), 0, s, a);
+      break;
+    case kFusedTz16:
+      hipLaunchKernelGGL(step_fused<16>, dim3(nblocks), dim3(kBlock), 0, s, a);
+      break;
+    case kSplitTz8:
+      hipLaunchKernelGGL(step_split_stress<8>, dim3(nblocks), dim3(kBlock), 0,
+                         s, a);
+      hipLaunchKernelGGL(step_split_velocity<8>, dim3(nblocks), dim3(kBlock),
+                         0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, long long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long long nb = (n + 255) / 256;
+  unsigned grid = (unsigned)(nb < 65536 ? nb : 65536);
+  hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_from_velocity(const double *vp, const double *vs,
+                                       const double *rho, double *lm,
+                                       double *mu, double *b, int nx, int nz,
+                                       int pitch, long long origin,
+                                       hipStream_t s) {
+  long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(update_from_velocity_kernel, dim3(blocks_for(n, 256)),
+                     dim3(256), 0, s, vp, vs, rho, lm, mu, b, nx, nz, pitch,
+                     origin);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_sources(const double *sx, const double *sz,
+                                  const double *ox, const double *oz,
+                                  double *ax, double *az, long long n,
+                                  hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(adjoint_sources_kernel, dim3(blocks_for(n, 256)),
+                     dim3(256), 0, s, sx, sz, ox, oz, ax, az, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_map_kernels(const double *vp, const double *vs,
+                              const double *rho, const double *lam_k,
+                              const double *mu_k, const double *rho_l,
+                              double *vp_k, double *vs_k, double *rho_v,
+                              long long n, hipStream_t s) {
+  hipLaunchKernelGGL(map_kernels_kernel, dim3(blocks_for(n, 256)), dim3(256),
+                     0, s, vp, vs, rho, lam_k, mu_k, rho_l, vp_k, vs_k, rho_v,
+                     n);
+  return hipGetLastError();
+}
+
+hipError_t launch_misfit(const double *sx, const double *sz, const double *ox,
+                         const double *oz, long long n, double half_dt,
+                         double *partials, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(misfit_partial_kernel, dim3(kMisfitBlocks), dim3(kBlock),
+                     0, s, sx, sz, ox, oz, n, half_dt, partials);
+  hipLaunchKernelGGL(misfit_final_kernel, dim3(1), dim3(kBlock), 0, s,
+                     partials, kMisfitBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_model_vector(int mode, double *vp, double *vs, double *rho,
+                               double *vec, int nx, int nxf, int nzf, int npx,
+                               int npz, int gx0, int gz0, hipStream_t s) {
+  int npp = (nxf / npx) * (nzf / npz);
+  hipLaunchKernelGGL(model_vector_kernel, dim3(blocks_for(npp, 256)),
+                     dim3(256), 0, s, mode, vp, vs, rho, vec, nx, nxf, nzf,
+                     npx, npz, gx0, gz0);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient_vector(const double *vp_k, const double *vs_k,
+                                  const double *rho_k, double *vec, int nx,
+                                  int nxf, int nzf, int npx, int npz, int gx0,
+                                  int gz0, hipStream_t s) {
+  int npp = (nxf / npx) * (nzf / npz);
+  hipLaunchKernelGGL(gradient_vector_kernel, dim3(blocks_for(npp, 256)),
+                     dim3(256), 0, s, vp_k, vs_k, rho_k, vec, nx, nxf, nzf,
+                     npx, npz, gx0, gz0);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
+                            int slices, int to_ref, hipStream_t s) {
+  int rows = to_ref ? nz : nx, cols = to_ref ? nx : nz;
+  for (int s0 = 0; s0 < slices; s0 += 65535) {
+    int cnt = slices - s0 < 65535 ? slices - s0 : 65535;
+    long long off = (long long)s0 * nx * nz;
+    dim3 grid((cols + 31) / 32, (rows + 31) / 32, cnt);
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, in + off,
+                       out + off, nx, nz, to_ref);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_unpad(const double *in, double *out, int nx, int nz,
+                        int pitch, long long origin, hipStream_t s) {
+  long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(unpad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     in, out, nx, nz, pitch, origin);
+  return hipGetLastError();
+}
+
+}  // namespace psv

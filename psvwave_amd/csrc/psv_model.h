@@ -1,0 +1,125 @@
+// psv_model.h -- device-resident fdModel: owns the HBM state of one model and
+// drives the time loops.  The C-ABI (psv_capi.hip) is a thin layer over it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "psv_config.h"
+#include "psv_kernels.h"
+
+namespace psv {
+
+struct RuntimeError : public std::runtime_error {
+  explicit RuntimeError(const std::string &m) : std::runtime_error(m) {}
+};
+
+class Model {
+ public:
+  explicit Model(const Config &c);
+  Model(const Model &other);  // deep copy of device state
+  ~Model();
+  Model &operator=(const Model &) = delete;
+
+  const Config &config() const { return cfg_; }
+  const Derived &derived() const { return d_; }
+  double misfit() const { return misfit_; }
+
+  // hot path (src/fdModel.cpp:526-969, 1128-1183, 1310-1324)
+  void forward(int i_shot, bool store_fields, bool verbose,
+               bool output_wavefields);
+  void adjoint(int i_shot, bool verbose);
+  double calculate_l2_misfit();
+  void calculate_l2_adjoint_sources();
+  void reset_kernels();
+  void map_kernels_to_velocity();
+  void run_model(bool verbose, bool simulate_adjoint);
+
+  // parameters and data, host buffers in the reference layout
+  void get_parameter_fields(double *vp, double *vs, double *rho) const;
+  void set_parameter_fields(const double *vp, const double *vs,
+                            const double *rho);
+  void get_kernels(double *vp_k, double *vs_k, double *rho_k) const;
+  void get_lame_kernels(double *lam_k, double *mu_k, double *rho_l) const;
+  void get_traces(int which, double *ux, double *uz) const;  // 0 syn 1 obs 2 adj
+  void set_traces(int which, const double *ux, const double *uz);
+  void get_snapshot(int i_shot, int i_snap, double *const out[5]) const;
+  void get_snapshots(double *const out[5]) const;
+  void get_model_vector(double *v) const;
+  void set_model_vector(const double *v);
+  void get_gradient_vector(double *v) const;
+  void get_setup(double *stf, double *moment, double *taper) const;
+
+  // text formats (src/fdModel.cpp:971-1126, 1185-1289, 1326-1356, 1517-1572)
+  void write_receivers() const;
+  void write_sources() const;
+  void write_kernels() const;
+  void load_receivers(bool verbose);
+  void load_model(const std::string &de, const std::string &vp,
+                  const std::string &vs, bool verbose);
+  std::vector<double> load_vector(const std::string &path, bool verbose) const;
+
+  // tuning and measurement
+  void set_step_variant(int variant, int xcd_remap);
+  void set_timing(int stride) { timing_stride_ = stride; }
+  void step_timing(double *ms, int64_t *timed, int64_t *launches) const {
+    *ms = timed_ms_;
+    *timed = timed_;
+    *launches = launches_;
+  }
+  void reset_step_timing() { timed_ms_ = 0, timed_ = 0, launches_ = 0; }
+  int64_t device_bytes() const { return device_bytes_; }
+
+ private:
+  void allocate();
+  void build_injection_tables();
+  void ensure_snapshots();
+  double *dalloc(size_t n);
+  void upload(double *dst, const double *src, size_t n) const;
+  void download(double *dst, const double *src, size_t n) const;
+  double *staging(size_t n) const;
+  void to_device_layout(double *dst_compact, const double *host_ref) const;
+  void to_host_layout(double *host_ref, const double *src_compact,
+                      int slices = 1) const;
+  StepArgs base_args() const;
+  void run_steps(StepArgs &a, int i_shot, bool forward, bool store,
+                 bool dump);
+  void sync() const;
+  void check_shot(int i_shot) const;
+
+  Config cfg_;
+  Derived d_;
+  int variant_ = kFusedTz8, xcd_remap_ = 1;
+  int tz_ = 8, ntx_ = 0, ntz_ = 0, ntiles_ = 0;
+  int pitch_ = 0, rows_ = 0;
+  long long origin_ = 0, npad_ = 0, ngrid_ = 0, nrec_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::vector<void *> allocations_;
+  int64_t device_bytes_ = 0;
+
+  double *state_[2][5] = {};  // ping-pong: vx, vz, txx, tzz, txz (padded)
+  double *lm_ = nullptr, *mu_ = nullptr, *b_ = nullptr;  // padded
+  double *vp_ = nullptr, *vs_ = nullptr, *rho_ = nullptr;  // compact
+  double *lam_k_ = nullptr, *mu_k_ = nullptr, *rho_l_ = nullptr;
+  double *vp_k_ = nullptr, *vs_k_ = nullptr, *rho_v_ = nullptr;
+  double *stf_ = nullptr, *taper_lut_ = nullptr;
+  double *syn_[2] = {}, *obs_[2] = {}, *adj_[2] = {};
+  double *snap_[5] = {};
+  long long *rec_idx_ = nullptr;
+  int *fwd_off_ = nullptr, *adj_off_ = nullptr;
+  InjEntry *fwd_entries_ = nullptr, *adj_entries_ = nullptr;
+  double *misfit_buf_ = nullptr;  // partials then the result
+  mutable double *staging_ = nullptr;
+  mutable size_t staging_n_ = 0;
+  double misfit_ = 0.0;
+
+  int timing_stride_ = 0;
+  double timed_ms_ = 0.0;
+  int64_t timed_ = 0, launches_ = 0;
+  std::vector<hipEvent_t> events_;
+};
+
+}  // namespace psv

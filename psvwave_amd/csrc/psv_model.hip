@@ -1,0 +1,788 @@
+// psv_model.hip -- host runtime of the MI355X fdModel.
+//
+// One Model owns every array of the reference class (src/fdModel.h:209-252)
+// in HBM, plus the injection tables and ping-pong state buffers the fused
+// step kernel needs.  All work is enqueued on the model's own stream; each
+// public call synchronises before returning.
+#include "psv_model.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <sstream>
+
+namespace psv {
+
+namespace {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw RuntimeError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+std::string zero_pad(int num, int pad) {
+  std::ostringstream ss;
+  ss << std::setw(pad) << std::setfill('0') << num;
+  return ss.str();
+}
+
+}  // namespace
+
+Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
+  allocate();
+  // homogeneous background then Lame fields (src/fdModel.cpp:248-261)
+  hip_check(launch_fill(vp_, ngrid_, cfg_.scalar_vp, stream_), "fill vp");
+  hip_check(launch_fill(vs_, ngrid_, cfg_.scalar_vs, stream_), "fill vs");
+  hip_check(launch_fill(rho_, ngrid_, cfg_.scalar_rho, stream_), "fill rho");
+  hip_check(launch_update_from_velocity(vp_, vs_, rho_, lm_, mu_, b_, d_.nx,
+                                        d_.nz, pitch_, origin_, stream_),
+            "update_from_velocity");
+  sync();
+}
+
+Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
+  variant_ = o.variant_;
+  xcd_remap_ = o.xcd_remap_;
+  allocate();
+  auto cp = [&](double *dst, const double *src, long long n) {
+    if (n > 0)
+      hip_check(hipMemcpyAsync(dst, src, n * sizeof(double),
+                               hipMemcpyDeviceToDevice, stream_),
+                "copy");
+  };
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 5; ++f) cp(state_[k][f], o.state_[k][f], npad_);
+  cp(lm_, o.lm_, npad_);
+  cp(mu_, o.mu_, npad_);
+  cp(b_, o.b_, npad_);
+  for (auto pr : {std::make_pair(vp_, o.vp_), std::make_pair(vs_, o.vs_),
+                  std::make_pair(rho_, o.rho_), std::make_pair(lam_k_, o.lam_k_),
+                  std::make_pair(mu_k_, o.mu_k_), std::make_pair(rho_l_, o.rho_l_),
+                  std::make_pair(vp_k_, o.vp_k_), std::make_pair(vs_k_, o.vs_k_),
+                  std::make_pair(rho_v_, o.rho_v_)})
+    cp(pr.first, pr.second, ngrid_);
+  for (int k = 0; k < 2; ++k) {
+    cp(syn_[k], o.syn_[k], nrec_);
+    cp(obs_[k], o.obs_[k], nrec_);
+    cp(adj_[k], o.adj_[k], nrec_);
+  }
+  if (o.snap_[0]) {
+    ensure_snapshots();
+    long long n = (long long)cfg_.n_shots * d_.snapshots * ngrid_;
+    for (int f = 0; f < 5; ++f) cp(snap_[f], o.snap_[f], n);
+  }
+  misfit_ = o.misfit_;
+  sync();
+}
+
+Model::~Model() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto e : events_) (void)hipEventDestroy(e);
+  for (void *p : allocations_) (void)hipFree(p);
+  if (staging_) (void)hipFree(staging_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double *Model::dalloc(size_t n) {
+  void *p = nullptr;
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(double);
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  allocations_.push_back(p);
+  device_bytes_ += (int64_t)bytes;
+  hip_check(launch_fill((double *)p, (long long)std::max<size_t>(n, 1), 0.0,
+                        stream_),
+            "zero-fill");
+  return (double *)p;
+}
+
+void Model::sync() const {
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+}
+
+void Model::allocate() {
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
+            "hipStreamCreate");
+  ngrid_ = (long long)d_.nx * d_.nz;
+  nrec_ = (long long)cfg_.n_shots * cfg_.nr * cfg_.nt;
+  // Padded stencil layout, large enough for any tile height in use.
+  ntx_ = (int)((d_.nx + kTileX - 1) / kTileX);
+  pitch_ = (int)round_up(kHaloX + (long long)ntx_ * kTileX + 4, 64);
+  rows_ = kHaloZ + (int)round_up(d_.nz, 16) + 4;
+  origin_ = (long long)kHaloZ * pitch_ + kHaloX;
+  npad_ = (long long)rows_ * pitch_;
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 5; ++f) state_[k][f] = dalloc(npad_);
+  lm_ = dalloc(npad_);
+  mu_ = dalloc(npad_);
+  b_ = dalloc(npad_);
+  vp_ = dalloc(ngrid_);
+  vs_ = dalloc(ngrid_);
+  rho_ = dalloc(ngrid_);
+  lam_k_ = dalloc(ngrid_);
+  mu_k_ = dalloc(ngrid_);
+  rho_l_ = dalloc(ngrid_);
+  vp_k_ = dalloc(ngrid_);
+  vs_k_ = dalloc(ngrid_);
+  rho_v_ = dalloc(ngrid_);
+  for (int k = 0; k < 2; ++k) {
+    syn_[k] = dalloc(nrec_);
+    obs_[k] = dalloc(nrec_);
+    adj_[k] = dalloc(nrec_);
+  }
+  stf_ = dalloc(d_.stf.size());
+  upload(stf_, d_.stf.data(), d_.stf.size());
+  taper_lut_ = dalloc(d_.taper_lut.size());
+  upload(taper_lut_, d_.taper_lut.data(), d_.taper_lut.size());
+  std::vector<long long> ridx(std::max(cfg_.nr, 1), 0);
+  for (int r = 0; r < cfg_.nr; ++r)
+    ridx[r] = origin_ + (long long)d_.iz_rec[r] * pitch_ + d_.ix_rec[r];
+  rec_idx_ = (long long *)dalloc(ridx.size());
+  sync();
+  hip_check(hipMemcpy(rec_idx_, ridx.data(), ridx.size() * sizeof(long long),
+                      hipMemcpyHostToDevice),
+            "upload receivers");
+  misfit_buf_ = dalloc(2 * kMisfitBlocks + 2);
+  build_injection_tables();
+  sync();
+}
+
+// Injection terms in the reference's order, bucketed by owning tile (stable,
+// so terms hitting one point keep their order).  Forward: 12 moment-tensor
+// terms per source (src/fdModel.cpp:681-751); adjoint: one vx and one vz
+// term per receiver (src/fdModel.cpp:951-958).
+void Model::build_injection_tables() {
+  tz_ = step_tile_height(variant_);
+  ntz_ = (d_.nz + tz_ - 1) / tz_;
+  ntiles_ = ntx_ * ntz_;
+  struct Term {
+    int x, z, comp, src;
+    double coef;
+  };
+  auto bucket = [&](const std::vector<Term> &terms, std::vector<int> &off,
+                    std::vector<InjEntry> &out, int base) {
+    std::vector<int> count(ntiles_ + 1, 0);
+    for (const Term &t : terms) count[(t.z / tz_) * ntx_ + t.x / kTileX]++;
+    off.assign(ntiles_ + 1, 0);
+    for (int i = 0; i < ntiles_; ++i) off[i + 1] = off[i] + count[i];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    std::vector<InjEntry> local(terms.size());
+    for (const Term &t : terms) {
+      int tile = (t.z / tz_) * ntx_ + t.x / kTileX;
+      InjEntry e;
+      e.local = (t.z % tz_) * kTileX + (t.x % kTileX);
+      e.comp = t.comp;
+      e.src = t.src;
+      e.pad = 0;
+      e.coef = t.coef;
+      local[fill[tile]++] = e;
+    }
+    for (auto &o : off) o += base;
+    out.insert(out.end(), local.begin(), local.end());
+  };
+
+  std::vector<int> fwd_off, off;
+  std::vector<InjEntry> fwd_entries;
+  for (int s = 0; s < cfg_.n_shots; ++s) {
+    std::vector<Term> terms;
+    for (int src : cfg_.which_source_to_fire_in_which_shot[s]) {
+      int x = d_.ix_src[src], z = d_.iz_src[src];
+      const double *M = &d_.moment[(size_t)src * 4];
+      double q01 = 0.25 * M[1], q10 = 0.25 * M[2];
+      terms.push_back({x - 1, z, 0, src, -M[0]});
+      terms.push_back({x, z, 0, src, M[0]});
+      terms.push_back({x, z - 1, 1, src, -M[3]});
+      terms.push_back({x, z, 1, src, M[3]});
+      terms.push_back({x - 1, z + 1, 0, src, q01});
+      terms.push_back({x, z + 1, 0, src, q01});
+      terms.push_back({x - 1, z - 1, 0, src, -q01});
+      terms.push_back({x, z - 1, 0, src, -q01});
+      terms.push_back({x + 1, z - 1, 1, src, q10});
+      terms.push_back({x + 1, z, 1, src, q10});
+      terms.push_back({x - 1, z - 1, 1, src, -q10});
+      terms.push_back({x - 1, z, 1, src, -q10});
+    }
+    bucket(terms, off, fwd_entries, (int)fwd_entries.size());
+    fwd_off.insert(fwd_off.end(), off.begin(), off.end());
+  }
+  std::vector<Term> aterms;
+  for (int r = 0; r < cfg_.nr; ++r) {
+    aterms.push_back({d_.ix_rec[r], d_.iz_rec[r], 0, r, 0.0});
+    aterms.push_back({d_.ix_rec[r], d_.iz_rec[r], 1, r, 0.0});
+  }
+  std::vector<int> adj_off;
+  std::vector<InjEntry> adj_entries;
+  bucket(aterms, adj_off, adj_entries, 0);
+
+  // (Re)upload; these small tables are reallocated when the tiling changes.
+  auto put = [&](void *&dst, const void *src, size_t bytes) {
+    if (dst) {
+      hip_check(hipStreamSynchronize(stream_), "sync");
+      auto it = std::find(allocations_.begin(), allocations_.end(), dst);
+      if (it != allocations_.end()) allocations_.erase(it);
+      hip_check(hipFree(dst), "hipFree");
+    }
+    dst = nullptr;
+    hip_check(hipMalloc(&dst, std::max<size_t>(bytes, 16)), "hipMalloc");
+    allocations_.push_back(dst);
+    if (bytes)
+      hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
+  };
+  if (fwd_off.empty()) fwd_off.assign(ntiles_ + 1, 0);
+  put((void *&)fwd_off_, fwd_off.data(), fwd_off.size() * sizeof(int));
+  put((void *&)fwd_entries_, fwd_entries.data(),
+      fwd_entries.size() * sizeof(InjEntry));
+  put((void *&)adj_off_, adj_off.data(), adj_off.size() * sizeof(int));
+  put((void *&)adj_entries_, adj_entries.data(),
+      adj_entries.size() * sizeof(InjEntry));
+}
+
+void Model::set_step_variant(int variant, int xcd_remap) {
+  if (variant < kFusedTz8 || variant > kSplitTz8)
+    throw ValueError("unknown step variant");
+  variant_ = variant;
+  xcd_remap_ = xcd_remap ? 1 : 0;
+  build_injection_tables();
+}
+
+void Model::ensure_snapshots() {
+  if (snap_[0]) return;
+  long long n = (long long)cfg_.n_shots * d_.snapshots * ngrid_;
+  for (int f = 0; f < 5; ++f) snap_[f] = dalloc(n);
+}
+
+void Model::upload(double *dst, const double *src, size_t n) const {
+  sync();  // fills and kernels on the model stream may still touch dst
+  if (n)
+    hip_check(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy H2D");
+}
+
+void Model::download(double *dst, const double *src, size_t n) const {
+  sync();
+  if (n)
+    hip_check(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost),
+              "hipMemcpy D2H");
+}
+
+double *Model::staging(size_t n) const {
+  if (n > staging_n_) {
+    sync();
+    if (staging_) (void)hipFree(staging_);
+    staging_ = nullptr;
+    hip_check(hipMalloc((void **)&staging_, n * sizeof(double)),
+              "hipMalloc staging");
+    staging_n_ = n;
+  }
+  return staging_;
+}
+
+void Model::to_device_layout(double *dst, const double *host_ref) const {
+  double *s = staging(ngrid_);
+  upload(s, host_ref, ngrid_);
+  hip_check(launch_transpose(s, dst, d_.nx, d_.nz, 1, 0, stream_), "transpose");
+  sync();
+}
+
+void Model::to_host_layout(double *host_ref, const double *src,
+                           int slices) const {
+  double *s = staging((size_t)ngrid_ * slices);
+  hip_check(launch_transpose(src, s, d_.nx, d_.nz, slices, 1, stream_),
+            "transpose");
+  download(host_ref, s, (size_t)ngrid_ * slices);
+}
+
+void Model::check_shot(int i_shot) const {
+  if (i_shot < 0 || i_shot >= cfg_.n_shots)
+    throw ValueError("i_shot " + std::to_string(i_shot) +
+                     " is outside [0, n_shots)");
+}
+
+StepArgs Model::base_args() const {
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nx = d_.nx;
+  a.nz = d_.nz;
+  a.pitch = pitch_;
+  a.ntx = ntx_;
+  a.ntz = ntz_;
+  a.tz = tz_;
+  a.origin = origin_;
+  a.lm = lm_;
+  a.mu = mu_;
+  a.b = b_;
+  a.taper_lut = taper_lut_;
+  a.np = cfg_.np_boundary;
+  a.dt = cfg_.dt;
+  a.dx = cfg_.dx;
+  a.dz = cfg_.dz;
+  a.dxdz = cfg_.dx * cfg_.dz;
+  a.d4x = cfg_.dx * cfg_.dx * cfg_.dx * cfg_.dx;
+  a.d4z = cfg_.dz * cfg_.dz * cfg_.dz * cfg_.dz;
+  a.nt = cfg_.nt;
+  a.lam_k = lam_k_;
+  a.mu_k = mu_k_;
+  a.rho_k = rho_l_;
+  a.kx0 = cfg_.np_boundary + cfg_.nx_inner_boundary;
+  a.kx1 = cfg_.np_boundary + cfg_.nx_inner - cfg_.nx_inner_boundary;
+  a.kz0 = cfg_.np_boundary + cfg_.nz_inner_boundary;
+  a.kz1 = cfg_.np_boundary + cfg_.nz_inner - cfg_.nz_inner_boundary;
+  a.wcorr = cfg_.snapshot_interval * cfg_.dt;
+  a.stf = stf_;
+  a.nr = cfg_.nr;
+  a.rec_idx = rec_idx_;
+  a.xcd_remap = xcd_remap_;
+  a.ntiles = ntiles_;
+  return a;
+}
+
+// The time loop: one step launch per time step (two for the split variant).
+void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
+                      bool dump) {
+  const bool inplace = step_is_inplace(variant_);
+  const int nt = cfg_.nt, si = cfg_.snapshot_interval;
+  // zero both state buffers (src/fdModel.cpp:530-542, 792-805)
+  for (int k = 0; k < (inplace ? 1 : 2); ++k)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+  size_t next_event = 0;
+  auto event = [&]() {
+    if (next_event == events_.size()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      events_.push_back(e);
+    }
+    return events_[next_event++];
+  };
+  for (int k = 0; k < nt; ++k) {
+    const int it = forward ? k : nt - 1 - k;
+    const int cur = inplace ? 0 : (k & 1), nxt = inplace ? 0 : ((k + 1) & 1);
+    a.vx_i = state_[cur][0];
+    a.vz_i = state_[cur][1];
+    a.txx_i = state_[cur][2];
+    a.tzz_i = state_[cur][3];
+    a.txz_i = state_[cur][4];
+    a.vx_o = state_[nxt][0];
+    a.vz_o = state_[nxt][1];
+    a.txx_o = state_[nxt][2];
+    a.tzz_o = state_[nxt][3];
+    a.txz_o = state_[nxt][4];
+    a.it = it;
+    a.do_snap = (it % si == 0) && (store || !forward);
+    if (a.do_snap) {
+      long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
+      for (int f = 0; f < 5; ++f) a.snap[f] = snap_[f] + off;
+    }
+    bool timed = timing_stride_ > 0 && (launches_ % timing_stride_) == 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+      e0 = event();
+      e1 = event();
+      hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    }
+    hip_check(launch_step(variant_, a, stream_), "step kernel");
+    if (timed) {
+      hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+      pairs.emplace_back(e0, e1);
+    }
+    ++launches_;
+    if (dump && it % 10 == 0) {
+      // src/fdModel.cpp:753-776: state after step it, default stream format
+      std::vector<double> hx(ngrid_), hz(ngrid_);
+      double *s = staging(ngrid_);
+      hip_check(launch_unpad(state_[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
+                             stream_),
+                "unpad");
+      to_host_layout(hx.data(), s);
+      hip_check(launch_unpad(state_[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
+                             stream_),
+                "unpad");
+      to_host_layout(hz.data(), s);
+      std::ofstream fx("snapshots/vx" + zero_pad(it, 5) + ".txt");
+      std::ofstream fz("snapshots/vz" + zero_pad(it, 5) + ".txt");
+      for (int ix = 0; ix < d_.nx; ++ix) {
+        for (int iz = 0; iz < d_.nz; ++iz) {
+          fx << hx[(size_t)ix * d_.nz + iz] << " ";
+          fz << hz[(size_t)ix * d_.nz + iz] << " ";
+        }
+        fx << std::endl;
+        fz << std::endl;
+      }
+    }
+  }
+  sync();
+  for (auto &p : pairs) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, p.first, p.second),
+              "hipEventElapsedTime");
+    timed_ms_ += ms;
+    ++timed_;
+  }
+}
+
+void Model::forward(int i_shot, bool store_fields, bool verbose,
+                    bool output_wavefields) {
+  check_shot(i_shot);
+  if (store_fields && cfg_.nt % cfg_.snapshot_interval != 0)
+    throw ValueError("Snapshot interval and size of accumulator don't match!"
+                     " (nt must be a multiple of snapshot_interval)");
+  if (store_fields) ensure_snapshots();
+  StepArgs a = base_args();
+  a.adjoint = 0;
+  a.dts = cfg_.dt;
+  a.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.entries = fwd_entries_;
+  a.rtf_x = syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.rtf_z = syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  auto t0 = std::chrono::steady_clock::now();
+  run_steps(a, i_shot, true, store_fields, output_wavefields);
+  if (verbose) {
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                   .count();
+    std::cout << "Seconds elapsed for forward wave simulation: " << s
+              << std::endl;
+  }
+}
+
+void Model::adjoint(int i_shot, bool verbose) {
+  check_shot(i_shot);
+  if (cfg_.nt % cfg_.snapshot_interval != 0)
+    throw ValueError("Snapshot interval and size of accumulator don't match!"
+                     " (nt must be a multiple of snapshot_interval)");
+  if (!snap_[0])
+    throw ValueError("adjoint_simulate needs stored forward wavefields; run "
+                     "forward_simulate with store_fields=True first");
+  StepArgs a = base_args();
+  a.adjoint = 1;
+  a.dts = -cfg_.dt;
+  a.tile_off = adj_off_;
+  a.entries = adj_entries_;
+  a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  auto t0 = std::chrono::steady_clock::now();
+  run_steps(a, i_shot, false, false, false);
+  if (verbose) {
+    double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                   .count();
+    std::cout << "Seconds elapsed for adjoint wave simulation: " << s
+              << std::endl;
+  }
+}
+
+double Model::calculate_l2_misfit() {
+  hip_check(launch_misfit(syn_[0], syn_[1], obs_[0], obs_[1], nrec_,
+                          0.5 * cfg_.dt, misfit_buf_,
+                          misfit_buf_ + 2 * kMisfitBlocks, stream_),
+            "misfit");
+  download(&misfit_, misfit_buf_ + 2 * kMisfitBlocks, 1);
+  return misfit_;
+}
+
+void Model::calculate_l2_adjoint_sources() {
+  hip_check(launch_adjoint_sources(syn_[0], syn_[1], obs_[0], obs_[1], adj_[0],
+                                   adj_[1], nrec_, stream_),
+            "adjoint sources");
+  sync();
+}
+
+void Model::reset_kernels() {
+  hip_check(launch_fill(lam_k_, ngrid_, 0.0, stream_), "reset");
+  hip_check(launch_fill(mu_k_, ngrid_, 0.0, stream_), "reset");
+  hip_check(launch_fill(rho_l_, ngrid_, 0.0, stream_), "reset");
+  sync();
+}
+
+void Model::map_kernels_to_velocity() {
+  hip_check(launch_map_kernels(vp_, vs_, rho_, lam_k_, mu_k_, rho_l_, vp_k_,
+                               vs_k_, rho_v_, ngrid_, stream_),
+            "map kernels");
+  sync();
+}
+
+// src/fdModel.cpp:1291-1308
+void Model::run_model(bool verbose, bool simulate_adjoint) {
+  for (int s = 0; s < cfg_.n_shots; ++s) forward(s, true, verbose, false);
+  calculate_l2_misfit();
+  if (simulate_adjoint) {
+    calculate_l2_adjoint_sources();
+    reset_kernels();
+    for (int s = 0; s < cfg_.n_shots; ++s) adjoint(s, verbose);
+    map_kernels_to_velocity();
+  }
+}
+
+void Model::get_parameter_fields(double *vp, double *vs, double *rho) const {
+  to_host_layout(vp, vp_);
+  to_host_layout(vs, vs_);
+  to_host_layout(rho, rho_);
+}
+
+void Model::set_parameter_fields(const double *vp, const double *vs,
+                                 const double *rho) {
+  to_device_layout(vp_, vp);
+  to_device_layout(vs_, vs);
+  to_device_layout(rho_, rho);
+  hip_check(launch_update_from_velocity(vp_, vs_, rho_, lm_, mu_, b_, d_.nx,
+                                        d_.nz, pitch_, origin_, stream_),
+            "update_from_velocity");
+  sync();
+}
+
+void Model::get_kernels(double *a, double *b, double *c) const {
+  to_host_layout(a, vp_k_);
+  to_host_layout(b, vs_k_);
+  to_host_layout(c, rho_v_);
+}
+
+void Model::get_lame_kernels(double *a, double *b, double *c) const {
+  to_host_layout(a, lam_k_);
+  to_host_layout(b, mu_k_);
+  to_host_layout(c, rho_l_);
+}
+
+void Model::get_traces(int which, double *ux, double *uz) const {
+  double *const *src = which == 0 ? syn_ : which == 1 ? obs_ : adj_;
+  download(ux, src[0], nrec_);
+  download(uz, src[1], nrec_);
+}
+
+void Model::set_traces(int which, const double *ux, const double *uz) {
+  double *const *dst = which == 0 ? syn_ : which == 1 ? obs_ : adj_;
+  upload(dst[0], ux, nrec_);
+  upload(dst[1], uz, nrec_);
+}
+
+void Model::get_snapshot(int i_shot, int i_snap, double *const out[5]) const {
+  check_shot(i_shot);
+  if (i_snap < 0 || i_snap >= d_.snapshots)
+    throw ValueError("snapshot index out of range");
+  if (!snap_[0]) {
+    for (int f = 0; f < 5; ++f) std::memset(out[f], 0, ngrid_ * sizeof(double));
+    return;
+  }
+  long long off = ((long long)i_shot * d_.snapshots + i_snap) * ngrid_;
+  for (int f = 0; f < 5; ++f) to_host_layout(out[f], snap_[f] + off);
+}
+
+void Model::get_snapshots(double *const out[5]) const {
+  long long slices = (long long)cfg_.n_shots * d_.snapshots;
+  if (!snap_[0]) {
+    for (int f = 0; f < 5; ++f)
+      std::memset(out[f], 0, slices * ngrid_ * sizeof(double));
+    return;
+  }
+  // bounded staging: about 512 MiB per transfer
+  long long chunk = std::max<long long>(1, (64ll << 20) / std::max<long long>(ngrid_, 1));
+  for (int f = 0; f < 5; ++f)
+    for (long long s0 = 0; s0 < slices; s0 += chunk) {
+      int cnt = (int)std::min(chunk, slices - s0);
+      to_host_layout(out[f] + s0 * ngrid_, snap_[f] + s0 * ngrid_, cnt);
+    }
+}
+
+void Model::get_model_vector(double *v) const {
+  double *s = staging(d_.free_parameters);
+  hip_check(launch_model_vector(0, vp_, vs_, rho_, s, d_.nx,
+                                d_.nx_free_parameters, d_.nz_free_parameters,
+                                cfg_.npx, cfg_.npz,
+                                cfg_.nx_inner_boundary + cfg_.np_boundary,
+                                cfg_.nz_inner_boundary + cfg_.np_boundary,
+                                stream_),
+            "model vector");
+  download(v, s, d_.free_parameters);
+}
+
+void Model::set_model_vector(const double *v) {
+  double *s = staging(d_.free_parameters);
+  upload(s, v, d_.free_parameters);
+  hip_check(launch_model_vector(1, vp_, vs_, rho_, s, d_.nx,
+                                d_.nx_free_parameters, d_.nz_free_parameters,
+                                cfg_.npx, cfg_.npz,
+                                cfg_.nx_inner_boundary + cfg_.np_boundary,
+                                cfg_.nz_inner_boundary + cfg_.np_boundary,
+                                stream_),
+            "model vector");
+  hip_check(launch_update_from_velocity(vp_, vs_, rho_, lm_, mu_, b_, d_.nx,
+                                        d_.nz, pitch_, origin_, stream_),
+            "update_from_velocity");
+  sync();
+}
+
+void Model::get_gradient_vector(double *v) const {
+  double *s = staging(d_.free_parameters);
+  hip_check(launch_gradient_vector(vp_k_, vs_k_, rho_v_, s, d_.nx,
+                                   d_.nx_free_parameters,
+                                   d_.nz_free_parameters, cfg_.npx, cfg_.npz,
+                                   cfg_.nx_inner_boundary + cfg_.np_boundary,
+                                   cfg_.nz_inner_boundary + cfg_.np_boundary,
+                                   stream_),
+            "gradient vector");
+  download(v, s, d_.free_parameters);
+}
+
+void Model::get_setup(double *stf, double *moment, double *taper) const {
+  if (stf) std::copy(d_.stf.begin(), d_.stf.end(), stf);
+  if (moment) std::copy(d_.moment.begin(), d_.moment.end(), moment);
+  if (taper)
+    for (int ix = 0; ix < d_.nx; ++ix)
+      for (int iz = 0; iz < d_.nz; ++iz)
+        taper[(size_t)ix * d_.nz + iz] =
+            d_.taper_lut[taper_count(ix, iz, d_.nx, d_.nz, cfg_.np_boundary)];
+}
+
+// src/fdModel.cpp:973-1006 (the prefix argument is not used there either)
+void Model::write_receivers() const {
+  std::vector<double> ux(nrec_), uz(nrec_);
+  get_traces(0, ux.data(), uz.data());
+  for (int s = 0; s < cfg_.n_shots; ++s) {
+    std::ofstream fx(cfg_.observed_data_folder + "/rtf_ux" + std::to_string(s) + ".txt");
+    std::ofstream fz(cfg_.observed_data_folder + "/rtf_uz" + std::to_string(s) + ".txt");
+    fx.precision(std::numeric_limits<double>::digits10 + 10);
+    fz.precision(std::numeric_limits<double>::digits10 + 10);
+    for (int r = 0; r < cfg_.nr; ++r) {
+      fx << std::endl;
+      fz << std::endl;
+      for (int it = 0; it < cfg_.nt; ++it) {
+        size_t k = ((size_t)s * cfg_.nr + r) * cfg_.nt + it;
+        fx << ux[k] << " ";
+        fz << uz[k] << " ";
+      }
+    }
+  }
+}
+
+// src/fdModel.cpp:1008-1032
+void Model::write_sources() const {
+  for (int s = 0; s < cfg_.n_shots; ++s) {
+    std::ofstream f(cfg_.stf_folder + "/sources_shot_" + std::to_string(s) + ".txt");
+    f.precision(std::numeric_limits<double>::digits10 + 10);
+    for (int src : cfg_.which_source_to_fire_in_which_shot[s]) {
+      f << std::endl;
+      for (int it = 0; it < cfg_.nt; ++it)
+        f << d_.stf[(size_t)src * cfg_.nt + it] << " ";
+    }
+  }
+}
+
+// src/fdModel.cpp:1326-1356
+void Model::write_kernels() const {
+  std::vector<double> a(ngrid_), b(ngrid_), c(ngrid_);
+  get_kernels(a.data(), b.data(), c.data());
+  std::ofstream fa("kernel_vp.txt"), fb("kernel_vs.txt"),
+      fc("kernel_density.txt");
+  for (int ix = 0; ix < d_.nx; ++ix) {
+    for (int iz = 0; iz < d_.nz; ++iz) {
+      size_t i = (size_t)ix * d_.nz + iz;
+      fa << a[i] << " ";
+      fb << b[i] << " ";
+      fc << c[i] << " ";
+    }
+    fa << std::endl;
+    fb << std::endl;
+    fc << std::endl;
+  }
+}
+
+// src/fdModel.cpp:1053-1126: stream extraction with the reference's
+// good()-state checks.
+void Model::load_receivers(bool verbose) {
+  std::vector<double> ux(nrec_), uz(nrec_);
+  for (int s = 0; s < cfg_.n_shots; ++s) {
+    std::string nx_ = cfg_.observed_data_folder + "/rtf_ux" + std::to_string(s) + ".txt";
+    std::string nz_ = cfg_.observed_data_folder + "/rtf_uz" + std::to_string(s) + ".txt";
+    std::ifstream fx(nx_), fz(nz_);
+    if (verbose) {
+      std::cout << "File for ux data at shot " << s << " is "
+                << (fx.good() ? "good (exists at least)." : "ungood.") << std::endl;
+      std::cout << "File for uz data at shot " << s << " is "
+                << (fz.good() ? "good (exists at least)." : "ungood.") << std::endl;
+    }
+    if (!fx.good() || !fz.good()) throw ValueError("Not all data is present!");
+    double px, pz;
+    for (int r = 0; r < cfg_.nr; ++r)
+      for (int it = 0; it < cfg_.nt; ++it) {
+        fx >> px;
+        fz >> pz;
+        size_t k = ((size_t)s * cfg_.nr + r) * cfg_.nt + it;
+        ux[k] = px;
+        uz[k] = pz;
+      }
+    if (!fx.good() || !fz.good()) throw ValueError("Not enough data is present!");
+    fx >> px;
+    fz >> pz;
+    if (fx.good() || fz.good()) throw ValueError("Too much data is present!");
+  }
+  set_traces(1, ux.data(), uz.data());
+}
+
+// src/fdModel.cpp:1185-1289
+void Model::load_model(const std::string &de, const std::string &vp,
+                       const std::string &vs, bool verbose) {
+  std::ifstream fd(de), fp(vp), fs(vs);
+  if (verbose) {
+    std::cout << "Loading models." << std::endl;
+    std::cout << "File: " << de << std::endl
+              << "File for density is "
+              << (fd.good() ? "good (exists at least)." : "ungood.") << std::endl;
+    std::cout << "File: " << vp << std::endl
+              << "File for P-wave velocity is "
+              << (fp.good() ? "good (exists at least)." : "ungood.") << std::endl;
+    std::cout << "File: " << vs << std::endl
+              << "File for S-wave velocity is "
+              << (fs.good() ? "good (exists at least)." : "ungood.") << std::endl;
+  }
+  if (!fd.good() || !fp.good() || !fs.good())
+    throw ValueError("The files for target models don't seem to exist.");
+  std::vector<double> hd(ngrid_), hp(ngrid_), hs(ngrid_);
+  for (int ix = 0; ix < d_.nx; ++ix)
+    for (int iz = 0; iz < d_.nz; ++iz) {
+      size_t i = (size_t)ix * d_.nz + iz;
+      fd >> hd[i];
+      fp >> hp[i];
+      fs >> hs[i];
+      if (!fd.good() || !fp.good() || !fs.good())
+        throw ValueError("Received bad state of one of the files at end of "
+                         "reading. Does the data match the domain?");
+    }
+  double extra;
+  fd >> extra;
+  fp >> extra;
+  fs >> extra;
+  if (fd.good() || fp.good() || fs.good())
+    throw ValueError("Too much data is present!");
+  set_parameter_fields(hp.data(), hs.data(), hd.data());
+}
+
+// src/fdModel.cpp:1517-1572 (size 3 * nxf * nzf regardless of the basis)
+std::vector<double> Model::load_vector(const std::string &path,
+                                       bool verbose) const {
+  size_t n = (size_t)3 * d_.nx_free_parameters * d_.nz_free_parameters;
+  std::vector<double> m(n);
+  std::ifstream f(path);
+  if (verbose) {
+    std::cout << "Loading model vector." << std::endl;
+    std::cout << "File: " << path << std::endl;
+    std::cout << "File is " << (f.good() ? "good (exists at least)." : "ungood.")
+              << std::endl;
+  }
+  if (!f.good())
+    throw ValueError("The files for target models don't seem to exist.");
+  double p;
+  for (size_t i = 0; i < n; ++i) {
+    f >> p;
+    m[i] = p;
+  }
+  if (!f.good()) throw ValueError("Not enough data is present!");
+  f >> p;
+  if (f.good()) throw ValueError("Too much data is present!");
+  return m;
+}
+
+}  // namespace psv
