@@ -1,0 +1,189 @@
+"""Benchmark: fwd+adj grid-point updates per second on one MI355X.
+
+Workload: the layered-faulted single model -- 1700 x 350 inner grid plus a
+50-point absorbing taper (1800 x 450 = 810,000 points), dx = dz = 5 m,
+dt = 0.3 ms, nt = 6000, FP64, 8 shots, 300 receivers, snapshots every 10
+steps.  One step = one FWI gradient: forward of all 8 shots with snapshot
+storage, L2 misfit, adjoint sources, kernel reset, adjoint of all 8 shots
+with kernel correlation, kernel mapping -- 7.776e10 point-updates.
+
+Prints one JSON line.  ``value`` is point-updates per second of the whole
+step with all inputs resident in HBM.  ``roofline`` prices the step kernel
+(the dominant kernel) at its algorithmic bytes per launch over its average
+launch duration, measured with HIP events on the model's stream.
+``cpu_baseline`` times the CPU oracle (OpenMP, all host cores up to 16) on a
+bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("fwd+adj Gpt-updates/s and % HBM peak; FWI time-to-misfit, "
+          "learned vs plain")
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def _args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=2)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--variant", type=int, default=0,
+                   help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split")
+    p.add_argument("--no-xcd-remap", action="store_true")
+    p.add_argument("--timing-stride", type=int, default=7,
+                   help="bracket every k-th step launch with HIP events")
+    p.add_argument("--nt", type=int, default=6000)
+    p.add_argument("--shots", type=int, default=8)
+    p.add_argument("--cpu-baseline-nt", type=int, default=400,
+                   help="time steps of the CPU sample (fwd + adj, 1 shot)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def cpu_baseline(nt_sample):
+    """CPU oracle on a bounded sample: one shot, forward + adjoint, the
+    benchmark grid, nt_sample steps."""
+    from oracle.pyoracle import OracleModel
+    from psvwave_amd.synthetic import layered_faulted_config, layered_faulted_model
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    OracleModel.set_threads(threads)
+    cfg = layered_faulted_config(nt=nt_sample, n_shots=1)
+    true = layered_faulted_model(cfg, faulted=False)
+    start = layered_faulted_model(cfg, faulted=True)
+    o = OracleModel(cfg)
+    o.set_parameter_fields(*true)
+    o.forward_simulate(0, store_fields=False)
+    o.set_observed_data(*o.get_synthetic_data())
+    o.set_parameter_fields(*start)
+    t0 = time.perf_counter()
+    o.forward_simulate(0)
+    o.calculate_l2_misfit()
+    o.calculate_l2_adjoint_sources()
+    o.reset_kernels()
+    o.adjoint_simulate(0)
+    o.map_kernels_to_velocity()
+    secs = time.perf_counter() - t0
+    points = 2.0 * o.nx * o.nz * nt_sample
+    return {"value": points / secs / 1e9, "unit": "Gpt-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": (f"1 shot fwd+adj, {o.nx}x{o.nz} grid, nt={nt_sample} "
+                       f"({points:.3e} point-updates, {secs:.2f} s, "
+                       f"OpenMP {threads} threads)")}
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; "
+                          "multi-GPU runs (shots sharded across GPUs) are not "
+                          "part of this build", "n_gpus": a.gpus}))
+        sys.exit(2)
+    import psvwave_amd
+    from psvwave_amd.synthetic import (algorithmic_bytes_per_shot,
+                                       layered_faulted_config,
+                                       layered_faulted_model)
+
+    cfg = layered_faulted_config(nt=a.nt, n_shots=a.shots)
+    model = psvwave_amd.fdModel(cfg)
+    model.set_step_variant(a.variant, not a.no_xcd_remap)
+    true = layered_faulted_model(cfg, faulted=False)
+    start = layered_faulted_model(cfg, faulted=True)
+    # observed data from the unfaulted model (setup, not timed)
+    model.set_parameter_fields(*true)
+    for s in range(model.n_shots):
+        model.forward_simulate(s, store_fields=False)
+    model.set_observed_data(*model.get_synthetic_data())
+    model.set_parameter_fields(*start)
+
+    def step():
+        for s in range(model.n_shots):
+            model.forward_simulate(s)
+        model.calculate_l2_misfit()
+        model.calculate_l2_adjoint_sources()
+        model.reset_kernels()
+        for s in range(model.n_shots):
+            model.adjoint_simulate(s)
+        model.map_kernels_to_velocity()
+
+    for _ in range(a.warmup):
+        step()
+    model.set_timing(a.timing_stride)
+    model.reset_step_timing()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()  # every call synchronises the model stream before returning
+    secs = time.perf_counter() - t0
+    kern_ms, timed, launches = model.step_timing()
+    model.set_timing(0)
+
+    nx, nz, nt = model.nx, model.nz, model.nt
+    points_per_step = 2.0 * model.n_shots * nx * nz * nt
+    value = points_per_step * a.steps / secs / 1e9
+    d = cfg["domain"]
+    kpts = (d["nx_inner"] - 2 * d["nx_inner_boundary"]) * \
+        (d["nz_inner"] - 2 * d["nz_inner_boundary"])
+    bytes_per_launch = algorithmic_bytes_per_shot(
+        nx, nz, nt, cfg["inversion"]["snapshot_interval"], kpts) / (2.0 * nt)
+    launches_per_kernel = 2 if a.variant == 2 else 1
+    avg_launch_s = kern_ms / 1e3 / max(timed, 1)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    kernel_name = {0: "step_fused<8>", 1: "step_fused<16>",
+                   2: "step_split_stress<8>+step_split_velocity<8>"}[a.variant]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded layered-faulted model, observed data from "
+                "the unfaulted model)",
+        "config": {
+            "workload": "layered-faulted single model, fwd+adj FWI gradient",
+            "grid": f"{d['nx_inner']}x{d['nz_inner']} inner + "
+                    f"{cfg['boundary']['np_boundary']} taper = {nx}x{nz}",
+            "nt": nt, "shots": model.n_shots, "receivers": model.nr,
+            "snapshot_interval": cfg["inversion"]["snapshot_interval"],
+            "point_updates_per_step": points_per_step,
+            "step_kernel": kernel_name,
+            "xcd_remap": not a.no_xcd_remap,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kernel_name,
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "timed_launches": timed,
+            "launches": launches,
+            "launches_per_time_step": launches_per_kernel,
+        },
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_baseline_nt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -583,6 +583,14 @@ void oracle_get_lame_kernels(const oracle_model *m, double *a, double *b,
   memcpy(c, m->rho_l_k, n);
 }
 
+void oracle_set_lame_kernels(oracle_model *m, const double *a, const double *b,
+                             const double *c) {
+  size_t n = (size_t)m->nx * m->nz * sizeof(double);
+  memcpy(m->lambda_k, a, n);
+  memcpy(m->mu_k, b, n);
+  memcpy(m->rho_l_k, c, n);
+}
+
 #define NREC_BYTES(m) ((size_t)(m)->n_shots * (m)->nr * (m)->nt * sizeof(double))
 void oracle_get_synthetic(const oracle_model *m, double *ux, double *uz) {
   memcpy(ux, m->rtf_ux, NREC_BYTES(m));

@@ -64,6 +64,8 @@ void oracle_get_kernels(const oracle_model *m, double *vp_k, double *vs_k,
                         double *rho_k);
 void oracle_get_lame_kernels(const oracle_model *m, double *lambda_k,
                              double *mu_k, double *rho_l_k);
+void oracle_set_lame_kernels(oracle_model *m, const double *lambda_k,
+                             const double *mu_k, const double *rho_l_k);
 void oracle_get_synthetic(const oracle_model *m, double *ux, double *uz);
 void oracle_set_synthetic(oracle_model *m, const double *ux, const double *uz);
 void oracle_get_observed(const oracle_model *m, double *ux, double *uz);

@@ -196,6 +196,10 @@ class OracleModel:
         self._lib.oracle_get_lame_kernels(self._h, *map(_dptr, a))
         return tuple(a)
 
+    def set_lame_kernels(self, lambda_k, mu_k, rho_l_k):
+        arrs = [_f64(a) for a in (lambda_k, mu_k, rho_l_k)]
+        self._lib.oracle_set_lame_kernels(self._h, *map(_dptr, arrs))
+
     def get_synthetic_data(self):
         a = self._rec()
         self._lib.oracle_get_synthetic(self._h, *map(_dptr, a))

@@ -160,6 +160,23 @@ def make_params(cfg):
     d, b, med = cfg["domain"], cfg["boundary"], cfg["medium"]
     s, r = cfg["sources"], cfg["receivers"]
     which = [list(w) for w in s["which_source_to_fire_in_which_shot"]]
+    # The C-ABI reads exactly the declared counts, so list lengths are
+    # checked here with the reference's messages (src/fdModel.cpp:414-452).
+    n_src = int(s["n_sources"])
+    if not (len(s["ix_sources"]) == len(s["iz_sources"]) ==
+            len(s["moment_angles"]) == n_src):
+        raise ValueError(
+            "Dimension mismatch between n_sources and sources.ix_sources, "
+            "sources.iz_sources or sources.moment_angles")
+    if len(which) != int(s["n_shots"]):
+        raise ValueError("Mismatch between n_shots and "
+                         "sources.which_source_to_fire_in_which_shot")
+    if sum(len(w) for w in which) != n_src:
+        raise ValueError("Mismatch between n_sources and "
+                         "sources.which_source_to_fire_in_which_shot.")
+    if not (len(r["ix_receivers"]) == len(r["iz_receivers"]) == int(r["nr"])):
+        raise ValueError("Mismatch between nr and receivers.ix_receivers or "
+                         "receivers.iz_receivers")
     offs = np.zeros(len(which) + 1, dtype=np.int32)
     for k, w in enumerate(which):
         offs[k + 1] = offs[k] + len(w)

@@ -1,0 +1,85 @@
+"""Seeded synthetic workloads for benchmarks and large-grid tests.
+
+``layered_faulted_config`` / ``layered_faulted_model`` build the single-model
+benchmark workload: a 1700 x 350 inner grid (plus a 50-point taper, dx = dz
+= 5 m, dt = 0.3 ms, nt = 6000, snapshot every 10 steps), a water layer over
+40 layers with vp in 1500-5500 m/s, vs = vp / 1.73, rho = 310 vp^0.25, cut by
+six normal faults; 8 shots of one 10 Hz Ricker source each and 300
+receivers.  Everything is drawn from ``numpy.random.default_rng(seed)``.
+"""
+import math
+
+import numpy as np
+
+WATER_ROWS = 20
+
+
+def layered_faulted_config(nt=6000, n_shots=8, nr=300, nx_inner=1700,
+                           nz_inner=350, np_boundary=50, snapshot_interval=10):
+    """Configuration dict (the layout of psvwave_amd.get_dictionary)."""
+    ix_src = [100 + 214 * k for k in range(n_shots)]
+    ix_rec = [int(math.floor(k * nx_inner / nr)) for k in range(nr)]
+    return {
+        "domain": dict(nt=nt, nx_inner=nx_inner, nz_inner=nz_inner,
+                       nx_inner_boundary=0, nz_inner_boundary=WATER_ROWS,
+                       dx=5.0, dz=5.0, dt=3e-4),
+        "boundary": dict(np_boundary=np_boundary, np_factor=0.015),
+        "medium": dict(scalar_rho=2000.0, scalar_vp=2500.0, scalar_vs=1445.0),
+        "sources": dict(peak_frequency=10.0, n_sources=n_shots, n_shots=n_shots,
+                        source_timeshift=0.0, delay_cycles_per_shot=0,
+                        moment_angles=[45.0] * n_shots, ix_sources=ix_src,
+                        iz_sources=[25] * n_shots,
+                        which_source_to_fire_in_which_shot=[[k] for k in range(n_shots)]),
+        "receivers": dict(nr=nr, ix_receivers=ix_rec, iz_receivers=[WATER_ROWS] * nr),
+        "inversion": dict(snapshot_interval=snapshot_interval),
+        "basis": dict(npx=1, npz=1),
+        "output": dict(observed_data_folder=".", stf_folder="."),
+    }
+
+
+def layered_faulted_model(cfg, seed=2, faulted=True, n_layers=40, n_faults=6):
+    """(vp, vs, rho) on the full (nx, nz) grid, reference layout.
+
+    Inner rows 0..19 are water (vp 1500, vs 0).  Below, n_layers layers with
+    sorted random interfaces and sorted vp; normal faults dipping 60 degrees
+    shift their hanging wall down by 5-25 rows (the seafloor stays flat).
+    The absorbing zone repeats the edge values.
+    """
+    d = cfg["domain"]
+    nxi, nzi = d["nx_inner"], d["nz_inner"]
+    npb = cfg["boundary"]["np_boundary"]
+    rng = np.random.default_rng(seed)
+    interfaces = np.sort(rng.choice(np.arange(WATER_ROWS + 1, nzi), n_layers - 1,
+                                    replace=False))
+    vps = np.sort(rng.uniform(1500.0, 5500.0, n_layers))
+    layer_of_row = np.searchsorted(interfaces, np.arange(nzi), side="right")
+    vp_col = vps[layer_of_row]
+    vp = np.tile(vp_col, (nxi, 1))  # (nx_inner, nz_inner)
+    faults = [(rng.uniform(100, 1600), int(rng.integers(5, 26)))
+              for _ in range(n_faults)]
+    if faulted:
+        z = np.arange(nzi)
+        x = np.arange(nxi)[:, None]
+        for x0, throw in faults:
+            xf = x0 + z / math.tan(math.radians(60.0))  # fault trace per row
+            hanging = x > xf[None, :]
+            src_rows = np.maximum(WATER_ROWS, z - throw)
+            shifted = vp[:, src_rows]
+            vp = np.where(hanging & (z[None, :] >= WATER_ROWS), shifted, vp)
+    vp[:, :WATER_ROWS] = 1500.0
+    vs = vp / 1.73
+    vs[:, :WATER_ROWS] = 0.0
+    rho = 310.0 * vp ** 0.25
+    pad = ((npb, npb), (npb, npb))
+    return tuple(np.ascontiguousarray(np.pad(a, pad, mode="edge"))
+                 for a in (vp, vs, rho))
+
+
+def algorithmic_bytes_per_shot(nx, nz, nt, si, kernel_points):
+    """Algorithmic HBM bytes of one shot's forward + adjoint: 104 B per
+    point-update (read vx, vz, txx, tzz, txz, lm, mu, b; write the five
+    fields), 40 B per grid point per forward snapshot, 88 B per kernel-region
+    point per adjoint correlation (read five snapshot values, read and write
+    three kernels)."""
+    n = nx * nz
+    return n * nt * 208 + (nt // si) * (40 * n + 88 * kernel_points)

@@ -1,0 +1,123 @@
+"""The reference's own tests, restated against the HIP fdModel.
+
+tests/test_file_constructor.cpp, test_variable_constructor.cpp,
+test_constructor_comparison.cpp, test_copy_constructor.cpp (bitwise-equal
+sums of rtf_ux), tests/python/test_copy.py and test_descent.py.
+"""
+import numpy as np
+import pytest
+
+from conftest import DEFAULT_INI, default_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _sum_rtf_ux(model):
+    ux, _ = model.get_synthetic_data()
+    return float(np.sum(ux))
+
+
+def _forward_all(model):
+    for s in range(model.n_shots):
+        model.forward_simulate(s, store_fields=False, verbose=True)
+
+
+def test_file_constructor_is_deterministic():
+    import psvwave_amd
+    m = psvwave_amd.fdModel(DEFAULT_INI)
+    sums = []
+    for _ in range(2):
+        _forward_all(m)
+        sums.append(_sum_rtf_ux(m))
+    assert sums[0] == sums[1]
+    assert sums[0] != 0.0
+
+
+def test_variable_constructor_is_deterministic():
+    import psvwave_amd
+    m = psvwave_amd.fdModel(default_config())
+    sums = []
+    for _ in range(2):
+        _forward_all(m)
+        sums.append(_sum_rtf_ux(m))
+    assert sums[0] == sums[1]
+
+
+def test_constructor_comparison():
+    import psvwave_amd
+    m1 = psvwave_amd.fdModel(default_config())
+    m2 = psvwave_amd.fdModel(DEFAULT_INI)
+    _forward_all(m1)
+    _forward_all(m2)
+    assert _sum_rtf_ux(m1) == _sum_rtf_ux(m2)
+    for a, b in zip(m1.get_synthetic_data(), m2.get_synthetic_data()):
+        assert np.array_equal(a, b)
+
+
+def test_copy_constructor():
+    import psvwave_amd
+    m1 = psvwave_amd.fdModel(DEFAULT_INI)
+    m2 = m1.copy()
+    _forward_all(m1)
+    _forward_all(m2)
+    assert _sum_rtf_ux(m1) == _sum_rtf_ux(m2)
+
+
+def test_copy():
+    import psvwave_amd
+    model = psvwave_amd.fdModel(DEFAULT_INI)
+    model2 = model.copy()
+    assert model2.nx == model.nx and model2.free_parameters == model.free_parameters
+
+
+def test_copy_modify():
+    import psvwave_amd
+    model1 = psvwave_amd.fdModel(DEFAULT_INI)
+    model2 = model1.copy()
+    model1.set_model_vector(model1.get_model_vector() + 1)
+    model2.set_model_vector(model2.get_model_vector() - 1)
+    assert np.any(model2.get_model_vector() != model1.get_model_vector())
+
+
+def test_descent():
+    """tests/python/test_descent.py: a -10% vp/vs circle, 3 steps of
+    m -= 0.25 g must lower the misfit."""
+    import psvwave_amd
+    model = psvwave_amd.fdModel(DEFAULT_INI)
+    IX, IZ = model.get_coordinates(True)
+    vp, vs, rho = model.get_parameter_fields()
+    vp_starting, vs_starting, rho_starting = vp, vs, rho
+    x_middle = (IX.max() + IX.min()) / 2
+    z_middle = (IZ.max() + IZ.min()) / 2
+    circle = ((IX - x_middle) ** 2 + (IZ - z_middle) ** 2) ** 0.5 < 15
+    model.set_parameter_fields(vp * (1 - 0.1 * circle), vs * (1 - 0.1 * circle), rho)
+    for i_shot in range(model.n_shots):
+        model.forward_simulate(i_shot, omp_threads_override=6)
+    ux_obs, uz_obs = model.get_synthetic_data()
+    model.set_observed_data(ux_obs, uz_obs)
+    model.set_parameter_fields(vp_starting, vs_starting, rho_starting)
+    for i_shot in range(model.n_shots):
+        model.forward_simulate(i_shot, omp_threads_override=6)
+    model.calculate_l2_misfit()
+    model.calculate_l2_adjoint_sources()
+    model.reset_kernels()
+    for i_shot in range(model.n_shots):
+        model.adjoint_simulate(i_shot, omp_threads_override=6)
+    model.map_kernels_to_velocity()
+    m = model.get_model_vector()
+    misfits = []
+    for _ in range(3):
+        g = model.get_gradient_vector()
+        m -= 0.25 * g
+        model.set_model_vector(m)
+        for i_shot in range(model.n_shots):
+            model.forward_simulate(i_shot, omp_threads_override=6)
+        model.calculate_l2_misfit()
+        model.calculate_l2_adjoint_sources()
+        misfits.append(model.misfit)
+        model.reset_kernels()
+        for i_shot in range(model.n_shots):
+            model.adjoint_simulate(i_shot, omp_threads_override=6)
+        model.map_kernels_to_velocity()
+    print("misfits", misfits)
+    assert misfits[0] > misfits[-1]
