@@ -42,10 +42,25 @@ def _args():
                    help="bracket every k-th step launch with HIP events")
     p.add_argument("--nt", type=int, default=6000)
     p.add_argument("--shots", type=int, default=8)
-    p.add_argument("--cpu-baseline-nt", type=int, default=400,
+    p.add_argument("--cpu-baseline-nt", type=int, default=1000,
                    help="time steps of the CPU sample (fwd + adj, 1 shot)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                   help="first run a plain copy kernel over BYTES (the "
+                        "reference point for byte counters) and report it")
+    p.add_argument("--only-calibrate", action="store_true")
     return p.parse_args()
+
+
+def calibrate_copy(nbytes, iters=5):
+    """Mean time of a plain device copy of nbytes; returns (ms, GB/s of
+    read + write traffic)."""
+    import ctypes as C
+    from psvwave_amd import _lib
+    lib = _lib.load()
+    ms = C.c_double()
+    _lib.check(lib.psv_calibrate_copy(int(nbytes), int(iters), C.byref(ms)))
+    return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def cpu_baseline(nt_sample):
@@ -90,6 +105,14 @@ def main():
                           "multi-GPU runs (shots sharded across GPUs) are not "
                           "part of this build", "n_gpus": a.gpus}))
         sys.exit(2)
+    copy = None
+    if a.calibrate_copy:
+        ms, gbs = calibrate_copy(a.calibrate_copy)
+        copy = {"bytes": a.calibrate_copy, "ms": round(ms, 4),
+                "read_write_GBs": round(gbs, 1)}
+        print(json.dumps({"copy_calibration": copy}), file=sys.stderr, flush=True)
+        if a.only_calibrate:
+            return
     import psvwave_amd
     from psvwave_amd.synthetic import (algorithmic_bytes_per_shot,
                                        layered_faulted_config,
@@ -180,6 +203,8 @@ def main():
             "launches_per_time_step": launches_per_kernel,
         },
     }
+    if copy:
+        out["roofline"]["copy_calibration"] = copy
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_baseline_nt)
     print(json.dumps(out), flush=True)

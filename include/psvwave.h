@@ -195,6 +195,10 @@ int psv_get_step_timing(psv_model *m, double *total_ms, int64_t *timed,
 int psv_reset_step_timing(psv_model *m);
 /* Device bytes held by the model. */
 int psv_device_bytes(const psv_model *m, int64_t *bytes);
+/* Streaming copy of `bytes` (a multiple of 16) device-to-device with a plain
+ * copy kernel, `iters` times; *ms is the mean kernel time (HIP events).
+ * Reference point for byte counters and sustainable bandwidth. */
+int psv_calibrate_copy(int64_t bytes, int iters, double *ms);
 
 #ifdef __cplusplus
 }

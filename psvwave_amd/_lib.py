@@ -110,6 +110,7 @@ _SIGNATURES = {
                             C.POINTER(C.c_int64)],
     "psv_reset_step_timing": [_P],
     "psv_device_bytes": [_P, C.POINTER(C.c_int64)],
+    "psv_calibrate_copy": [C.c_int64, _i, _D],
 }
 _RESTYPES = {"psv_last_error": C.c_char_p, "psv_config_destroy": None,
              "psv_destroy": None}

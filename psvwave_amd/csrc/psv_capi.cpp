@@ -551,4 +551,47 @@ int psv_device_bytes(const psv_model *m, int64_t *bytes) {
   });
 }
 
+int psv_calibrate_copy(int64_t bytes, int iters, double *ms) {
+  return guard([&] {
+    need(ms, "ms");
+    if (bytes <= 0 || bytes % 16 != 0 || iters < 1)
+      throw psv::ValueError("bytes must be a positive multiple of 16, iters >= 1");
+    auto chk = [](hipError_t e, const char *w) {
+      if (e != hipSuccess) throw psv::RuntimeError(std::string(w) + ": " + hipGetErrorString(e));
+    };
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+      if (s) (void)hipStreamSynchronize(s);
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (s) (void)hipStreamDestroy(s);
+    };
+    try {
+      long long n = bytes / 8;
+      chk(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+      chk(hipMalloc((void **)&a, bytes), "hipMalloc");
+      chk(hipMalloc((void **)&b, bytes), "hipMalloc");
+      chk(psv::launch_fill(a, n, 1.0, s), "fill");
+      chk(psv::launch_copy(a, b, n, s), "copy");  // warm up
+      chk(hipEventCreate(&e0), "hipEventCreate");
+      chk(hipEventCreate(&e1), "hipEventCreate");
+      chk(hipEventRecord(e0, s), "hipEventRecord");
+      for (int i = 0; i < iters; ++i) chk(psv::launch_copy(a, b, n, s), "copy");
+      chk(hipEventRecord(e1, s), "hipEventRecord");
+      chk(hipEventSynchronize(e1), "hipEventSynchronize");
+      float t = 0.f;
+      chk(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+      *ms = t / iters;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 }  // extern "C"

@@ -172,6 +172,10 @@ hipError_t launch_gradient_vector(const double *vp_k, const double *vs_k,
 // reference's [slice][ix][iz].  to_ref = 1: device -> reference.
 hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
                             int slices, int to_ref, hipStream_t s);
+// Streaming copy of n doubles (n even), for bandwidth and counter
+// calibration.
+hipError_t launch_copy(const double *in, double *out, long long n,
+                       hipStream_t s);
 // Padded stencil array -> compact [iz][ix].
 hipError_t launch_unpad(const double *in, double *out, int nx, int nz,
                         int pitch, long long origin, hipStream_t s);

@@ -449,6 +449,16 @@ __global__ void unpad_kernel(const double *in, double *out, int nx, int nz,
   out[i] = in[origin + (long long)z * pitch + x];
 }
 
+// Plain streaming copy, the reference point for byte counters and for the
+// sustainable bandwidth of a read+write stream.
+__global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ in,
+                                                   double2 *__restrict__ out,
+                                                   long long n2) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n2; i += stride) out[i] = in[i];
+}
+
 inline unsigned blocks_for(long long n, int bs) {
   return (unsigned)((n + bs - 1) / bs);
 }
@@ -561,6 +571,16 @@ hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
     hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, s, in + off,
                        out + off, nx, nz, to_ref);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(const double *in, double *out, long long n,
+                       hipStream_t s) {
+  long long n2 = n / 2;
+  long long nb = (n2 + 255) / 256;
+  unsigned grid = (unsigned)(nb < 8192 ? nb : 8192);
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s,
+                     (const double2 *)in, (double2 *)out, n2);
   return hipGetLastError();
 }
 
