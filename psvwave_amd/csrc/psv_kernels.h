@@ -48,6 +48,7 @@ struct StepArgs {
   int np;
   double dts;  // +dt forward, -dt adjoint (stencil sign)
   double dt, dx, dz, dxdz, d4x, d4z;
+  double rdx, rdz;  // RN(1/dx), RN(1/dz) for RecipDiv
   int it, nt, adjoint;
   // snapshot slot of this step (compact [iz][ix]); written forward, read
   // adjoint when do_snap is set
@@ -74,35 +75,59 @@ struct StepArgs {
 // trees follow the reference operation by operation; with contraction off
 // the results are bitwise those of the reference's double build.
 //
+// Division by a grid spacing.  IeeeDiv is the plain IEEE quotient.
+// RecipDiv multiplies by y = RN(1/d) and corrects once with an FMA
+// (Markstein): for |x| in [2^-960, 2^1000] the result is the correctly
+// rounded quotient, i.e. bit-identical to x / d; zeros keep their sign via
+// x * y, and every other x (tiny, huge, inf, NaN) takes the IEEE division.
+struct IeeeDiv {
+  double d;
+  __host__ __device__ double operator()(double x) const { return x / d; }
+};
+struct RecipDiv {
+  double d, y;
+  __host__ __device__ double operator()(double x) const {
+    double q = x * y;
+    double r = __builtin_fma(-q, d, x);
+    double res = __builtin_fma(r, y, q);
+    double ax = __builtin_fabs(x);
+    if (x == 0.0) res = q;
+    if (!(ax >= 0x1p-960 && ax <= 0x1p1000) && x != 0.0) res = x / d;
+    return res;
+  }
+};
+
 // Stress at one point from velocities v*, accessed at offsets sx (one column)
 // and sz (one row).  src/fdModel.cpp:615-642 (forward), :882-909 (adjoint).
+template <class DX, class DZ>
 __host__ __device__ inline void stress_point(const double *vx,
                                              const double *vz, long long i,
                                              long long sx, long long sz,
                                              double lm, double mu,
                                              double taper, double dts,
-                                             double dx, double dz, double &txx,
+                                             DX divx, DZ divz, double &txx,
                                              double &tzz, double &txz) {
   double dvx_x = kC1 * (vx[i + sx] - vx[i]) + kC2 * (vx[i - sx] - vx[i + 2 * sx]);
   double dvz_z = kC1 * (vz[i] - vz[i - sz]) + kC2 * (vz[i - 2 * sz] - vz[i + sz]);
   double dvx_z = kC1 * (vx[i + sz] - vx[i]) + kC2 * (vx[i - sz] - vx[i + 2 * sz]);
   double dvz_x = kC1 * (vz[i] - vz[i - sx]) + kC2 * (vz[i - 2 * sx] - vz[i + sx]);
   double la = lm - 2 * mu;  // la array of src/fdModel.cpp:1046, recomputed
-  txx = taper * (txx + dts * (lm * dvx_x / dx + la * dvz_z / dz));
-  tzz = taper * (tzz + dts * (la * dvx_x / dx + lm * dvz_z / dz));
-  txz = taper * (txz + dts * mu * (dvx_z / dz + dvz_x / dx));
+  txx = taper * (txx + dts * (divx(lm * dvx_x) + divz(la * dvz_z)));
+  tzz = taper * (tzz + dts * (divx(la * dvx_x) + divz(lm * dvz_z)));
+  txz = taper * (txz + dts * mu * (divz(dvx_z) + divx(dvz_x)));
 }
 
 // Velocity at one point from stresses.  src/fdModel.cpp:661-676 (forward),
 // :931-946 (adjoint); b_vx == b_vz == 1/rho (src/fdModel.cpp:1047-1048).
+template <class DX, class DZ>
 __host__ __device__ inline void velocity_point(
     const double *txx, const double *tzz, const double *txz, long long i,
-    long long sx, long long sz, double b, double taper, double dts, double dx,
-    double dz, double &vx, double &vz) {
-  double ax = (kC1 * (txx[i] - txx[i - sx]) + kC2 * (txx[i - 2 * sx] - txx[i + sx])) / dx +
-              (kC1 * (txz[i] - txz[i - sz]) + kC2 * (txz[i - 2 * sz] - txz[i + sz])) / dz;
-  double az = (kC1 * (txz[i + sx] - txz[i]) + kC2 * (txz[i - sx] - txz[i + 2 * sx])) / dx +
-              (kC1 * (tzz[i + sz] - tzz[i]) + kC2 * (tzz[i - sz] - tzz[i + 2 * sz])) / dz;
+    long long sx, long long sz, double b, double taper, double dts, DX divx,
+    DZ divz, double &vx, double &vz) {
+  double ax = divx(kC1 * (txx[i] - txx[i - sx]) + kC2 * (txx[i - 2 * sx] - txx[i + sx])) +
+              divz(kC1 * (txz[i] - txz[i - sz]) + kC2 * (txz[i - 2 * sz] - txz[i + sz]));
+  double az = divx(kC1 * (txz[i + sx] - txz[i]) + kC2 * (txz[i - sx] - txz[i + 2 * sx])) +
+              divz(kC1 * (tzz[i + sz] - tzz[i]) + kC2 * (tzz[i - sz] - tzz[i + 2 * sz]));
   vx = taper * (vx + b * dts * ax);
   vz = taper * (vz + b * dts * az);
 }
@@ -133,8 +158,17 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
   return d + 1 < np ? d + 1 : np;
 }
 
-// Step-kernel variants.
-enum StepVariant { kFusedTz8 = 0, kFusedTz16 = 1, kSplitTz8 = 2 };
+// Step-kernel variants: 0-2 divide by the spacings with IEEE divisions,
+// 3-5 are the same kernels with RecipDiv (bit-identical results).
+enum StepVariant {
+  kFusedTz8 = 0,
+  kFusedTz16 = 1,
+  kSplitTz8 = 2,
+  kFusedTz8Recip = 3,
+  kFusedTz16Recip = 4,
+  kSplitTz8Recip = 5,
+  kNumVariants = 6
+};
 
 // Host launchers (psv_kernels.hip).  All enqueue on `stream`.
 hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream);

@@ -112,8 +112,22 @@ __device__ inline void owner_snapshot(const StepArgs &a, int x, int z,
   }
 }
 
-template <int TZ>
+template <bool RECIP>
+struct Divs;
+template <>
+struct Divs<false> {
+  IeeeDiv x, z;
+  __device__ explicit Divs(const StepArgs &a) : x{a.dx}, z{a.dz} {}
+};
+template <>
+struct Divs<true> {
+  RecipDiv x, z;
+  __device__ explicit Divs(const StepArgs &a) : x{a.dx, a.rdx}, z{a.dz, a.rdz} {}
+};
+
+template <int TZ, bool RECIP>
 __global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
+  const Divs<RECIP> dv(a);
   constexpr int WV = kTileX + 8, HV = TZ + 8;  // velocity halo region
   constexpr int WS = kTileX + 4, HS = TZ + 4;  // stress ring region
   __shared__ double s_vx[HV * WV], s_vz[HV * WV];
@@ -154,7 +168,7 @@ __global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
       }
       if (interior(a, x, z)) {
         stress_point(s_vx, s_vz, (long long)(r + 2) * WV + (c + 2), 1, WV,
-                     a.lm[g], a.mu[g], taper_at(a, x, z), a.dts, a.dx, a.dz,
+                     a.lm[g], a.mu[g], taper_at(a, x, z), a.dts, dv.x, dv.z,
                      txx, tzz, txz);
       }
       if (own) {
@@ -189,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
     double b = a.b[g];
     if (interior(a, x, z)) {
       velocity_point(s_txx, s_tzz, s_txz, (long long)(r + 2) * WS + (c + 2), 1,
-                     WS, b, taper_at(a, x, z), a.dts, a.dx, a.dz, vx, vz);
+                     WS, b, taper_at(a, x, z), a.dts, dv.x, dv.z, vx, vz);
     }
     apply_injection(a, tile, r * kTileX + c, b, vx, vz);
     a.vx_o[g] = vx;
@@ -199,8 +213,9 @@ __global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
 
 // Split variant, launch 1 of 2: start-of-step owner work plus the in-place
 // stress update.  Velocities are read-only here.
-template <int TZ>
+template <int TZ, bool RECIP>
 __global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
+  const Divs<RECIP> dv(a);
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
@@ -215,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
       owner_snapshot(a, x, z, a.vx_i[g], a.vz_i[g], txx, tzz, txz);
     if (interior(a, x, z)) {
       stress_point(a.vx_i, a.vz_i, g, 1, a.pitch, a.lm[g], a.mu[g],
-                   taper_at(a, x, z), a.dts, a.dx, a.dz, txx, tzz, txz);
+                   taper_at(a, x, z), a.dts, dv.x, dv.z, txx, tzz, txz);
       a.txx_o[g] = txx;
       a.tzz_o[g] = tzz;
       a.txz_o[g] = txz;
@@ -224,8 +239,9 @@ __global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
 }
 
 // Split variant, launch 2 of 2: in-place velocity update and injection.
-template <int TZ>
+template <int TZ, bool RECIP>
 __global__ void __launch_bounds__(kBlock) step_split_velocity(StepArgs a) {
+  const Divs<RECIP> dv(a);
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
@@ -237,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) step_split_velocity(StepArgs a) {
     double vx = a.vx_i[g], vz = a.vz_i[g], b = a.b[g];
     if (interior(a, x, z)) {
       velocity_point(a.txx_o, a.tzz_o, a.txz_o, g, 1, a.pitch, b,
-                     taper_at(a, x, z), a.dts, a.dx, a.dz, vx, vz);
+                     taper_at(a, x, z), a.dts, dv.x, dv.z, vx, vz);
     }
     apply_injection(a, tile, r * kTileX + c, b, vx, vz);
     a.vx_o[g] = vx;
@@ -465,23 +481,36 @@ inline unsigned blocks_for(long long n, int bs) {
 
 }  // namespace
 
-int step_tile_height(int variant) { return variant == kFusedTz16 ? 16 : 8; }
-bool step_is_inplace(int variant) { return variant == kSplitTz8; }
+int step_tile_height(int variant) {
+  return (variant == kFusedTz16 || variant == kFusedTz16Recip) ? 16 : 8;
+}
+bool step_is_inplace(int variant) {
+  return variant == kSplitTz8 || variant == kSplitTz8Recip;
+}
 
 hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s) {
   int nblocks = a.xcd_remap ? 8 * ((a.ntiles + 7) / 8) : a.ntiles;
+  dim3 g(nblocks), b(kBlock);
   switch (variant) {
     case kFusedTz8:
-      hipLaunchKernelGGL(step_fused<8>, dim3(nblocks), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL((step_fused<8, false>), g, b, 0, s, a);
       break;
     case kFusedTz16:
-      hipLaunchKernelGGL(step_fused<16>, dim3(nblocks), dim3(kBlock), 0, s, a);
+      hipLaunchKernelGGL((step_fused<16, false>), g, b, 0, s, a);
       break;
     case kSplitTz8:
-      hipLaunchKernelGGL(step_split_stress<8>, dim3(nblocks), dim3(kBlock), 0,
-                         s, a);
-      hipLaunchKernelGGL(step_split_velocity<8>, dim3(nblocks), dim3(kBlock),
-                         0, s, a);
+      hipLaunchKernelGGL((step_split_stress<8, false>), g, b, 0, s, a);
+      hipLaunchKernelGGL((step_split_velocity<8, false>), g, b, 0, s, a);
+      break;
+    case kFusedTz8Recip:
+      hipLaunchKernelGGL((step_fused<8, true>), g, b, 0, s, a);
+      break;
+    case kFusedTz16Recip:
+      hipLaunchKernelGGL((step_fused<16, true>), g, b, 0, s, a);
+      break;
+    case kSplitTz8Recip:
+      hipLaunchKernelGGL((step_split_stress<8, true>), g, b, 0, s, a);
+      hipLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, a);
       break;
     default:
       return hipErrorInvalidValue;

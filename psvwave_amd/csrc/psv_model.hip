@@ -244,7 +244,7 @@ void Model::build_injection_tables() {
 }
 
 void Model::set_step_variant(int variant, int xcd_remap) {
-  if (variant < kFusedTz8 || variant > kSplitTz8)
+  if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
@@ -325,6 +325,8 @@ StepArgs Model::base_args() const {
   a.dxdz = cfg_.dx * cfg_.dz;
   a.d4x = cfg_.dx * cfg_.dx * cfg_.dx * cfg_.dx;
   a.d4z = cfg_.dz * cfg_.dz * cfg_.dz * cfg_.dz;
+  a.rdx = 1.0 / cfg_.dx;
+  a.rdz = 1.0 / cfg_.dz;
   a.nt = cfg_.nt;
   a.lam_k = lam_k_;
   a.mu_k = mu_k_;

@@ -370,7 +370,9 @@ class fdModel:
         return stf, mom, taper
 
     def set_step_variant(self, variant, xcd_remap=True):
-        """0: fused step kernel, 64x8 tiles; 1: fused, 64x16; 2: split."""
+        """Step kernel: 0 fused 64x8 tiles, 1 fused 64x16, 2 split (two
+        launches per step); 3-5 the same with reciprocal-multiply division
+        by the grid spacings (bit-identical results)."""
         _lib.check(self._lib.psv_set_step_variant(self._h, int(variant),
                                                   int(bool(xcd_remap))))
 

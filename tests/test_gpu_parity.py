@@ -114,7 +114,7 @@ def _small_models(cfg):
     return true, start
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 def test_small_chain_matches_oracle(oracle_lib, variant):
     cfg = small_config()
     true, start = _small_models(cfg)
