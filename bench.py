@@ -39,11 +39,13 @@ def _args():
                    help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split; "
                         "3-5 the same with reciprocal-multiply division")
     p.add_argument("--no-xcd-remap", action="store_true")
+    p.add_argument("--graph", action="store_true",
+                   help="replay each shot's time loop as a captured HIP graph")
     p.add_argument("--timing-stride", type=int, default=7,
                    help="bracket every k-th step launch with HIP events")
     p.add_argument("--nt", type=int, default=6000)
     p.add_argument("--shots", type=int, default=8)
-    p.add_argument("--cpu-baseline-nt", type=int, default=1000,
+    p.add_argument("--cpu-baseline-nt", type=int, default=4000,
                    help="time steps of the CPU sample (fwd + adj, 1 shot)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -122,6 +124,8 @@ def main():
     cfg = layered_faulted_config(nt=a.nt, n_shots=a.shots)
     model = psvwave_amd.fdModel(cfg)
     model.set_step_variant(a.variant, not a.no_xcd_remap)
+    if a.graph:
+        model.set_graph_mode(True)
     true = layered_faulted_model(cfg, faulted=False)
     start = layered_faulted_model(cfg, faulted=True)
     # observed data from the unfaulted model (setup, not timed)
@@ -191,6 +195,7 @@ def main():
             "point_updates_per_step": points_per_step,
             "step_kernel": kernel_name,
             "xcd_remap": not a.no_xcd_remap,
+            "hip_graph": a.graph,
         },
         "roofline": {
             "bound": "hbm",

@@ -190,6 +190,10 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * and the number of bracketed launches since the last reset. */
 int psv_set_step_variant(psv_model *m, int variant, int xcd_remap);
 int psv_set_timing(psv_model *m, int timing_stride);
+/* on = 1: capture each (shot, direction, store_fields) time loop once as a
+ * HIP graph and replay it; with timing enabled, each replay is bracketed as
+ * a whole (kernel time plus inter-kernel gaps). */
+int psv_set_graph_mode(psv_model *m, int on);
 int psv_get_step_timing(psv_model *m, double *total_ms, int64_t *timed,
                         int64_t *launches);
 int psv_reset_step_timing(psv_model *m);

@@ -523,6 +523,10 @@ int psv_set_step_variant(psv_model *m, int variant, int xcd_remap) {
   return guard([&] { M(m).set_step_variant(variant, xcd_remap); });
 }
 
+int psv_set_graph_mode(psv_model *m, int on) {
+  return guard([&] { M(m).set_graph_mode(on != 0); });
+}
+
 int psv_set_timing(psv_model *m, int stride) {
   return guard([&] {
     if (stride < 0) throw psv::ValueError("timing stride must be >= 0");

@@ -171,7 +171,8 @@ enum StepVariant {
 };
 
 // Host launchers (psv_kernels.hip).  All enqueue on `stream`.
-hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream);
+hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int step_tile_height(int variant);
 bool step_is_inplace(int variant);
 

@@ -20,6 +20,8 @@
 // independent cross-check of the fused kernel and as a tuning alternative.
 #include "psv_kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace psv {
 
 namespace {
@@ -125,66 +127,108 @@ struct Divs<true> {
   __device__ explicit Divs(const StepArgs &a) : x{a.dx, a.rdx}, z{a.dz, a.rdz} {}
 };
 
-template <int TZ, bool RECIP>
-__global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
-  const Divs<RECIP> dv(a);
+template <int TZ, bool RECIP, int NT = kBlock>
+__global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
   constexpr int WV = kTileX + 8, HV = TZ + 8;  // velocity halo region
   constexpr int WS = kTileX + 4, HS = TZ + 4;  // stress ring region
+  constexpr int NB = (HS * WS + NT - 1) / NT;  // stress items per thread
+  constexpr int NC = (TZ * kTileX + NT - 1) / NT;  // velocity items per thread
   __shared__ double s_vx[HV * WV], s_vz[HV * WV];
   __shared__ double s_txx[HS * WS], s_tzz[HS * WS], s_txz[HS * WS];
+  const Divs<RECIP> dv(a);
 
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
   const int tid = threadIdx.x;
 
-  // phase A: velocities of the tile with a 4-point halo (always inside the
-  // zero margin of the padded arrays).
-  for (int i = tid; i < HV * WV; i += kBlock) {
+  // phase A: issue every global load of the block up front.  Velocities
+  // with a 4-point halo (inside the zero margin) and stresses on the ring go
+  // to LDS; materials of this thread's stress and velocity items go to
+  // registers.  Points outside the grid stage zeros.
+  for (int i = tid; i < HV * WV; i += NT) {
     int r = i / WV, c = i - r * WV;
     long long g = pidx(a, x0 - 4 + c, z0 - 4 + r);
     s_vx[i] = a.vx_i[g];
     s_vz[i] = a.vz_i[g];
   }
+  double lm_r[NB], mu_r[NB], b_r[NC];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    lm_r[k] = 0.0;
+    mu_r[k] = 0.0;
+    if (i < HS * WS) {
+      int r = i / WS, c = i - r * WS;
+      int x = x0 - 2 + c, z = z0 - 2 + r;
+      double txx = 0.0, tzz = 0.0, txz = 0.0;
+      if (x >= 0 && x < a.nx && z >= 0 && z < a.nz) {
+        long long g = pidx(a, x, z);
+        txx = a.txx_i[g];
+        tzz = a.tzz_i[g];
+        txz = a.txz_i[g];
+        if (interior(a, x, z)) {
+          lm_r[k] = a.lm[g];
+          mu_r[k] = a.mu[g];
+        }
+      }
+      s_txx[i] = txx;
+      s_tzz[i] = tzz;
+      s_txz[i] = txz;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    int i = tid + k * NT;
+    b_r[k] = 0.0;
+    if (i < TZ * kTileX) {
+      int r = i / kTileX, c = i - r * kTileX;
+      int x = x0 + c, z = z0 + r;
+      if (x < a.nx && z < a.nz) b_r[k] = a.b[pidx(a, x, z)];
+    }
+  }
   if (!a.adjoint && tile == 0) record_receivers(a);
   __syncthreads();
 
-  // phase B: stresses on the tile plus a 2-point ring.
-  for (int i = tid; i < HS * WS; i += kBlock) {
+  // phase B: stresses on the tile plus a 2-point ring, in place in LDS (a
+  // stress update reads only its own old stress).
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    if (i >= HS * WS) break;
     int r = i / WS, c = i - r * WS;
     int x = x0 - 2 + c, z = z0 - 2 + r;
-    double txx = 0.0, tzz = 0.0, txz = 0.0;
-    if (x >= 0 && x < a.nx && z >= 0 && z < a.nz) {
-      long long g = pidx(a, x, z);
-      txx = a.txx_i[g];
-      tzz = a.tzz_i[g];
-      txz = a.txz_i[g];
-      bool own = r >= 2 && r < TZ + 2 && c >= 2 && c < kTileX + 2;
-      if (own && a.do_snap && !a.adjoint) {
-        long long ci = (long long)z * a.nx + x;
-        a.snap[2][ci] = txx;
-        a.snap[3][ci] = tzz;
-        a.snap[4][ci] = txz;
-      }
-      if (interior(a, x, z)) {
-        stress_point(s_vx, s_vz, (long long)(r + 2) * WV + (c + 2), 1, WV,
-                     a.lm[g], a.mu[g], taper_at(a, x, z), a.dts, dv.x, dv.z,
-                     txx, tzz, txz);
-      }
-      if (own) {
-        a.txx_o[g] = txx;
-        a.tzz_o[g] = tzz;
-        a.txz_o[g] = txz;
-      }
+    if (x < 0 || x >= a.nx || z < 0 || z >= a.nz) continue;
+    double txx = s_txx[i], tzz = s_tzz[i], txz = s_txz[i];
+    bool own = r >= 2 && r < TZ + 2 && c >= 2 && c < kTileX + 2;
+    if (own && a.do_snap && !a.adjoint) {
+      long long ci = (long long)z * a.nx + x;
+      a.snap[2][ci] = txx;
+      a.snap[3][ci] = tzz;
+      a.snap[4][ci] = txz;
     }
-    s_txx[i] = txx;
-    s_tzz[i] = tzz;
-    s_txz[i] = txz;
+    if (interior(a, x, z)) {
+      stress_point(s_vx, s_vz, (long long)(r + 2) * WV + (c + 2), 1, WV,
+                   lm_r[k], mu_r[k], taper_at(a, x, z), a.dts, dv.x, dv.z,
+                   txx, tzz, txz);
+      s_txx[i] = txx;
+      s_tzz[i] = tzz;
+      s_txz[i] = txz;
+    }
+    if (own) {
+      long long g = pidx(a, x, z);
+      a.txx_o[g] = txx;
+      a.tzz_o[g] = tzz;
+      a.txz_o[g] = txz;
+    }
   }
   __syncthreads();
 
   // phase C: velocities on the tile.
-  for (int i = tid; i < TZ * kTileX; i += kBlock) {
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    int i = tid + k * NT;
+    if (i >= TZ * kTileX) break;
     int r = i / kTileX, c = i - r * kTileX;
     int x = x0 + c, z = z0 + r;
     if (x >= a.nx || z >= a.nz) continue;
@@ -200,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) step_fused(StepArgs a) {
         owner_snapshot(a, x, z, vx, vz, a.txx_i[g], a.tzz_i[g], a.txz_i[g]);
       }
     }
-    double b = a.b[g];
+    double b = b_r[k];
     if (interior(a, x, z)) {
       velocity_point(s_txx, s_tzz, s_txz, (long long)(r + 2) * WS + (c + 2), 1,
                      WS, b, taper_at(a, x, z), a.dts, dv.x, dv.z, vx, vz);
@@ -466,13 +510,19 @@ __global__ void unpad_kernel(const double *in, double *out, int nx, int nz,
 }
 
 // Plain streaming copy, the reference point for byte counters and for the
-// sustainable bandwidth of a read+write stream.
+// sustainable bandwidth of a read+write stream: each thread keeps four
+// independent 16-byte loads in flight.
 __global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ in,
                                                    double2 *__restrict__ out,
                                                    long long n2) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n2; i += stride) out[i] = in[i];
+  long long base = (long long)blockIdx.x * 1024 + threadIdx.x;
+  double2 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n2) v[k] = in[base + k * 256];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k * 256 < n2) out[base + k * 256] = v[k];
 }
 
 inline unsigned blocks_for(long long n, int bs) {
@@ -488,29 +538,33 @@ bool step_is_inplace(int variant) {
   return variant == kSplitTz8 || variant == kSplitTz8Recip;
 }
 
-hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s) {
+// With e0/e1 non-null the launch records the kernel's own start and end
+// timestamps into them (hipExtLaunchKernel), i.e. the kernel duration without
+// the queueing around it.
+hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
+                       hipEvent_t e0, hipEvent_t e1) {
   int nblocks = a.xcd_remap ? 8 * ((a.ntiles + 7) / 8) : a.ntiles;
-  dim3 g(nblocks), b(kBlock);
+  dim3 g(nblocks), b(kBlock), b512(512);
   switch (variant) {
     case kFusedTz8:
-      hipLaunchKernelGGL((step_fused<8, false>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_fused<8, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kFusedTz16:
-      hipLaunchKernelGGL((step_fused<16, false>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_fused<16, false, 512>), g, b512, 0, s, e0, e1, 0, a);
       break;
     case kSplitTz8:
-      hipLaunchKernelGGL((step_split_stress<8, false>), g, b, 0, s, a);
-      hipLaunchKernelGGL((step_split_velocity<8, false>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_split_stress<8, false>), g, b, 0, s, e0, nullptr, 0, a);
+      hipExtLaunchKernelGGL((step_split_velocity<8, false>), g, b, 0, s, nullptr, e1, 0, a);
       break;
     case kFusedTz8Recip:
-      hipLaunchKernelGGL((step_fused<8, true>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_fused<8, true>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kFusedTz16Recip:
-      hipLaunchKernelGGL((step_fused<16, true>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_fused<16, true, 512>), g, b512, 0, s, e0, e1, 0, a);
       break;
     case kSplitTz8Recip:
-      hipLaunchKernelGGL((step_split_stress<8, true>), g, b, 0, s, a);
-      hipLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, a);
+      hipExtLaunchKernelGGL((step_split_stress<8, true>), g, b, 0, s, e0, nullptr, 0, a);
+      hipExtLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, nullptr, e1, 0, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -606,8 +660,7 @@ hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
 hipError_t launch_copy(const double *in, double *out, long long n,
                        hipStream_t s) {
   long long n2 = n / 2;
-  long long nb = (n2 + 255) / 256;
-  unsigned grid = (unsigned)(nb < 8192 ? nb : 8192);
+  unsigned grid = (unsigned)((n2 + 1023) / 1024);
   hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s,
                      (const double2 *)in, (double2 *)out, n2);
   return hipGetLastError();

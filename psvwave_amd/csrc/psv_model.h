@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,9 @@ class Model {
   // tuning and measurement
   void set_step_variant(int variant, int xcd_remap);
   void set_timing(int stride) { timing_stride_ = stride; }
+  // Capture each (direction, shot, store) time loop once as a HIP graph and
+  // replay it; step timing then brackets whole loops.
+  void set_graph_mode(bool on);
   void step_timing(double *ms, int64_t *timed, int64_t *launches) const {
     *ms = timed_ms_;
     *timed = timed_;
@@ -120,6 +124,10 @@ class Model {
   double timed_ms_ = 0.0;
   int64_t timed_ = 0, launches_ = 0;
   std::vector<hipEvent_t> events_;
+  bool graph_mode_ = false;
+  std::map<long long, hipGraphExec_t> graphs_;
+  void clear_graphs();
+  void enqueue_steps(StepArgs &a, int i_shot, bool forward, bool store);
 };
 
 }  // namespace psv

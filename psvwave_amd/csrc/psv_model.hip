@@ -84,6 +84,7 @@ Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
 
 Model::~Model() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  clear_graphs();
   for (auto e : events_) (void)hipEventDestroy(e);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
@@ -248,11 +249,23 @@ void Model::set_step_variant(int variant, int xcd_remap) {
     throw ValueError("unknown step variant");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
+  clear_graphs();  // graphs hold pointers into the injection tables
   build_injection_tables();
+}
+
+void Model::clear_graphs() {
+  for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+}
+
+void Model::set_graph_mode(bool on) {
+  if (!on) clear_graphs();
+  graph_mode_ = on;
 }
 
 void Model::ensure_snapshots() {
   if (snap_[0]) return;
+  clear_graphs();
   long long n = (long long)cfg_.n_shots * d_.snapshots * ngrid_;
   for (int f = 0; f < 5; ++f) snap_[f] = dalloc(n);
 }
@@ -344,12 +357,94 @@ StepArgs Model::base_args() const {
   return a;
 }
 
-// The time loop: one step launch per time step (two for the split variant).
-void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
-                      bool dump) {
+// Enqueues one time loop: zero the state, then one step launch per time step
+// (two for the split variants).  No synchronisation.
+void Model::enqueue_steps(StepArgs &a, int i_shot, bool forward, bool store) {
   const bool inplace = step_is_inplace(variant_);
   const int nt = cfg_.nt, si = cfg_.snapshot_interval;
-  // zero both state buffers (src/fdModel.cpp:530-542, 792-805)
+  // zero the state (src/fdModel.cpp:530-542, 792-805) with a fill kernel,
+  // which is safe inside a captured graph
+  for (int k = 0; k < (inplace ? 1 : 2); ++k)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
+  for (int k = 0; k < nt; ++k) {
+    const int it = forward ? k : nt - 1 - k;
+    const int cur = inplace ? 0 : (k & 1), nxt = inplace ? 0 : ((k + 1) & 1);
+    a.vx_i = state_[cur][0];
+    a.vz_i = state_[cur][1];
+    a.txx_i = state_[cur][2];
+    a.tzz_i = state_[cur][3];
+    a.txz_i = state_[cur][4];
+    a.vx_o = state_[nxt][0];
+    a.vz_o = state_[nxt][1];
+    a.txx_o = state_[nxt][2];
+    a.tzz_o = state_[nxt][3];
+    a.txz_o = state_[nxt][4];
+    a.it = it;
+    a.do_snap = (it % si == 0) && (store || !forward);
+    if (a.do_snap) {
+      long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
+      for (int f = 0; f < 5; ++f) a.snap[f] = snap_[f] + off;
+    }
+    hip_check(launch_step(variant_, a, stream_), "step kernel");
+  }
+}
+
+// The time loop.  Plain mode launches step by step (with optional HIP-event
+// brackets around every timing_stride-th launch, and the reference's text
+// dump); graph mode replays a captured graph of the whole loop.
+void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
+                      bool dump) {
+  const int nt = cfg_.nt;
+  if (graph_mode_ && !dump) {
+    long long key = ((long long)i_shot * 2 + (forward ? 1 : 0)) * 2 + (store ? 1 : 0);
+    auto it = graphs_.find(key);
+    if (it == graphs_.end()) {
+      sync();
+      hipGraph_t graph = nullptr;
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture");
+      try {
+        enqueue_steps(a, i_shot, forward, store);
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(stream_, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hip_check(e, "hipGraphInstantiate");
+      it = graphs_.emplace(key, exec).first;
+    }
+    bool timed = timing_stride_ > 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+      while (events_.size() < 2) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        events_.push_back(e);
+      }
+      e0 = events_[0];
+      e1 = events_[1];
+      hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
+    }
+    hip_check(hipGraphLaunch(it->second, stream_), "hipGraphLaunch");
+    if (timed) hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
+    sync();
+    launches_ += nt;
+    if (timed) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      timed_ms_ += ms;  // includes the state fills and inter-kernel gaps
+      timed_ += nt;
+    }
+    return;
+  }
+  const bool inplace = step_is_inplace(variant_);
+  const int si = cfg_.snapshot_interval;
   for (int k = 0; k < (inplace ? 1 : 2); ++k)
     for (int f = 0; f < 5; ++f)
       hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
@@ -387,13 +482,9 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
     if (timed) {
       e0 = event();
       e1 = event();
-      hip_check(hipEventRecord(e0, stream_), "hipEventRecord");
-    }
-    hip_check(launch_step(variant_, a, stream_), "step kernel");
-    if (timed) {
-      hip_check(hipEventRecord(e1, stream_), "hipEventRecord");
       pairs.emplace_back(e0, e1);
     }
+    hip_check(launch_step(variant_, a, stream_, e0, e1), "step kernel");
     ++launches_;
     if (dump && it % 10 == 0) {
       // src/fdModel.cpp:753-776: state after step it, default stream format

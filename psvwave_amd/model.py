@@ -376,6 +376,10 @@ class fdModel:
         _lib.check(self._lib.psv_set_step_variant(self._h, int(variant),
                                                   int(bool(xcd_remap))))
 
+    def set_graph_mode(self, on=True):
+        """Replay each shot's time loop as a captured HIP graph."""
+        _lib.check(self._lib.psv_set_graph_mode(self._h, int(bool(on))))
+
     def set_timing(self, stride):
         """Bracket every `stride`-th step launch with HIP events (0: off)."""
         _lib.check(self._lib.psv_set_timing(self._h, int(stride)))
