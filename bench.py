@@ -37,7 +37,8 @@ def _args():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--variant", type=int, default=0,
                    help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split; "
-                        "3-5 the same with reciprocal-multiply division")
+                        "3-5 the same with reciprocal-multiply division; "
+                        "6, 7 two steps per launch (64x8, 64x16)")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
@@ -164,13 +165,14 @@ def main():
         (d["nz_inner"] - 2 * d["nz_inner_boundary"])
     bytes_per_launch = algorithmic_bytes_per_shot(
         nx, nz, nt, cfg["inversion"]["snapshot_interval"], kpts) / (2.0 * nt)
-    launches_per_kernel = 2 if a.variant in (2, 5) else 1
+    launches_per_kernel = {2: 2, 5: 2, 6: 0.5, 7: 0.5}.get(a.variant, 1)
     avg_launch_s = kern_ms / 1e3 / max(timed, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     kernel_name = {0: "step_fused<8,ieee>", 1: "step_fused<16,ieee>",
                    2: "step_split_stress<8,ieee>+step_split_velocity<8,ieee>",
                    3: "step_fused<8,recip>", 4: "step_fused<16,recip>",
                    5: "step_split_stress<8,recip>+step_split_velocity<8,recip>",
+                   6: "step_two<8,256>", 7: "step_two<16,512>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -16,8 +16,8 @@
 namespace psv {
 
 constexpr int kTileX = 64;   // tile width = one wavefront of lanes
-constexpr int kHaloX = 4;    // margin columns left of x = 0
-constexpr int kHaloZ = 4;    // margin rows above z = 0
+constexpr int kHaloX = 8;    // margin columns left of x = 0
+constexpr int kHaloZ = 8;    // margin rows above z = 0
 constexpr int kBlock = 256;  // threads per block for the step kernels
 
 // 4th-order staggered coefficients (src/fdModel.h:201-202).
@@ -67,8 +67,19 @@ struct StepArgs {
   int nr;
   const long long *rec_idx;      // padded index of each receiver
   double *rtf_x, *rtf_z;
+  int rec_increments;            // store dt v / (dx dz) only; a scan follows
   int xcd_remap;                 // map consecutive tiles to one XCD's L2
   int ntiles;
+  // two-step launches: the second step of the pair
+  int it1, do_snap1;
+  double *snap1[5];
+  // injection terms of every point of a tile's 4-point halo region (CSR by
+  // tile, local index over the (64 + 8) x (tz + 8) region)
+  const int *halo_off;
+  const InjEntry *halo_entries;
+  // receivers by owning tile (forward record of two-step launches)
+  const int *rec_off;
+  const InjEntry *rec_entries;
 };
 
 // Pointwise arithmetic, written once for the device kernels.  Expression
@@ -159,7 +170,8 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 }
 
 // Step-kernel variants: 0-2 divide by the spacings with IEEE divisions,
-// 3-5 are the same kernels with RecipDiv (bit-identical results).
+// 3-5 are the same kernels with RecipDiv (bit-identical results), 6-7
+// advance two time steps per launch.
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -167,7 +179,9 @@ enum StepVariant {
   kFusedTz8Recip = 3,
   kFusedTz16Recip = 4,
   kSplitTz8Recip = 5,
-  kNumVariants = 6
+  kTwoStepTz8 = 6,
+  kTwoStepTz16 = 7,
+  kNumVariants = 8
 };
 
 // Host launchers (psv_kernels.hip).  All enqueue on `stream`.
@@ -175,6 +189,15 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 int step_tile_height(int variant);
 bool step_is_inplace(int variant);
+bool step_is_two_step(int variant);
+// The single-step variant a two-step variant uses for an odd last step.
+int step_single_variant(int variant);
+// Two time steps (a.it, a.it1) in one launch; variant 6 or 7.
+hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t stream,
+                            hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Running sums of recorded receiver increments, in place: [nr][nt] x 2.
+hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
+                                hipStream_t s);
 
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
 hipError_t launch_update_from_velocity(const double *vp, const double *vs,

@@ -57,7 +57,7 @@ __device__ inline void record_receivers(const StepArgs &a) {
     double incx = a.dt * a.vx_i[li] / a.dxdz;
     double incz = a.dt * a.vz_i[li] / a.dxdz;
     long long k = (long long)r * a.nt + a.it;
-    if (a.it == 0) {
+    if (a.it == 0 || a.rec_increments) {
       a.rtf_x[k] = incx;
       a.rtf_z[k] = incz;
     } else {
@@ -67,20 +67,24 @@ __device__ inline void record_receivers(const StepArgs &a) {
   }
 }
 
-// Owner-side application of this tile's injection terms, in list order.
-__device__ inline void apply_injection(const StepArgs &a, int tile, int local,
-                                       double b, double &vx, double &vz) {
-  int e0 = a.tile_off[tile], e1 = a.tile_off[tile + 1];
+// Application of a tile's injection terms to one point, in list order:
+// entries[off[tile] .. off[tile+1]) whose local index is `local`, at time
+// step it.
+__device__ inline void apply_terms(const StepArgs &a, const int *off,
+                                   const InjEntry *entries, int tile,
+                                   int local, int it, double b, double &vx,
+                                   double &vz) {
+  int e0 = off[tile], e1 = off[tile + 1];
   for (int e = e0; e < e1; ++e) {
-    const InjEntry &E = a.entries[e];
+    const InjEntry &E = entries[e];
     if (E.local != local) continue;
     double term;
     if (!a.adjoint) {
-      double S = a.stf[(long long)E.src * a.nt + a.it];
+      double S = a.stf[(long long)E.src * a.nt + it];
       term = E.coef * S * a.dt * b / (E.comp == 0 ? a.d4x : a.d4z);
     } else {
       const double *src = E.comp == 0 ? a.adj_x : a.adj_z;
-      term = a.dt * b * src[(long long)E.src * a.nt + a.it] / a.dxdz;
+      term = a.dt * b * src[(long long)E.src * a.nt + it] / a.dxdz;
     }
     if (E.comp == 0)
       vx += term;
@@ -89,25 +93,48 @@ __device__ inline void apply_injection(const StepArgs &a, int tile, int local,
   }
 }
 
+// Owner-side application of this tile's injection terms at step a.it.
+__device__ inline void apply_injection(const StepArgs &a, int tile, int local,
+                                       double b, double &vx, double &vz) {
+  apply_terms(a, a.tile_off, a.entries, tile, local, a.it, b, vx, vz);
+}
+
+// Forward receiver record by the owner of a point: dt v / (dx dz) at step it
+// for every receiver entry at `local` (two-step launches; a scan forms the
+// running sums afterwards).
+__device__ inline void record_owned(const StepArgs &a, int tile, int local,
+                                    int it, double vx, double vz) {
+  int e0 = a.rec_off[tile], e1 = a.rec_off[tile + 1];
+  for (int e = e0; e < e1; ++e) {
+    const InjEntry &E = a.rec_entries[e];
+    if (E.local != local) continue;
+    long long k = (long long)E.src * a.nt + it;
+    if (E.comp == 0)
+      a.rtf_x[k] = a.dt * vx / a.dxdz;
+    else
+      a.rtf_z[k] = a.dt * vz / a.dxdz;
+  }
+}
+
 // Everything the owner of (x, z) does with the state at the start of the
 // step: snapshot store (forward, src/fdModel.cpp:555-572) or correlation
 // (adjoint, src/fdModel.cpp:817-861).
-__device__ inline void owner_snapshot(const StepArgs &a, int x, int z,
-                                      double vx, double vz, double txx,
-                                      double tzz, double txz) {
+__device__ inline void owner_snapshot(const StepArgs &a, double *const *snap,
+                                      int x, int z, double vx, double vz,
+                                      double txx, double tzz, double txz) {
   long long ci = (long long)z * a.nx + x;
   if (!a.adjoint) {
-    a.snap[0][ci] = vx;
-    a.snap[1][ci] = vz;
-    a.snap[2][ci] = txx;
-    a.snap[3][ci] = tzz;
-    a.snap[4][ci] = txz;
+    snap[0][ci] = vx;
+    snap[1][ci] = vz;
+    snap[2][ci] = txx;
+    snap[3][ci] = tzz;
+    snap[4][ci] = txz;
   } else if (x >= a.kx0 && x < a.kx1 && z >= a.kz0 && z < a.kz1) {
     long long pi = pidx(a, x, z);
     double rk = a.rho_k[ci], lk = a.lam_k[ci], mk = a.mu_k[ci];
     correlate_point(a.wcorr, a.lm[pi], a.mu[pi], vx, vz, txx, tzz, txz,
-                    a.snap[0][ci], a.snap[1][ci], a.snap[2][ci], a.snap[3][ci],
-                    a.snap[4][ci], rk, lk, mk);
+                    snap[0][ci], snap[1][ci], snap[2][ci], snap[3][ci],
+                    snap[4][ci], rk, lk, mk);
     a.rho_k[ci] = rk;
     a.lam_k[ci] = lk;
     a.mu_k[ci] = mk;
@@ -241,7 +268,7 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
         a.snap[0][ci] = vx;
         a.snap[1][ci] = vz;
       } else {
-        owner_snapshot(a, x, z, vx, vz, a.txx_i[g], a.tzz_i[g], a.txz_i[g]);
+        owner_snapshot(a, a.snap, x, z, vx, vz, a.txx_i[g], a.tzz_i[g], a.txz_i[g]);
       }
     }
     double b = b_r[k];
@@ -252,6 +279,253 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
     apply_injection(a, tile, r * kTileX + c, b, vx, vz);
     a.vx_o[g] = vx;
     a.vz_o[g] = vz;
+  }
+}
+
+
+// Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
+// velocities with an 8-point halo and stresses with a 6-point ring, then
+// updates in place in LDS: stresses of step s0 on the 6-ring, velocities on
+// the 4-ring (applying step s0's injection terms on the whole 4-ring, so the
+// halo copies are exact), stresses of step s1 on the 2-ring, velocities of
+// step s1 on the tile.  Global traffic per point-update is about half that
+// of a one-step launch; the stencil work on the rings is recomputed.
+template <int TZ, int NT>
+__global__ void __launch_bounds__(NT) step_two(StepArgs a) {
+  constexpr int TX = kTileX;
+  constexpr int WV = TX + 16, HV = TZ + 16;  // v on the 8-ring
+  constexpr int WS = TX + 12, HS = TZ + 12;  // stresses on the 6-ring
+  constexpr int W4 = TX + 8, H4 = TZ + 8;    // velocity items of step s0
+  constexpr int W2 = TX + 4, H2 = TZ + 4;    // stress items of step s1
+  constexpr int NB = (HS * WS + NT - 1) / NT;
+  constexpr int NC = (H4 * W4 + NT - 1) / NT;
+  constexpr int ND = (H2 * W2 + NT - 1) / NT;
+  constexpr int NE = (TZ * TX + NT - 1) / NT;
+  __shared__ double s_vx[HV * WV], s_vz[HV * WV];
+  __shared__ double s_txx[HS * WS], s_tzz[HS * WS], s_txz[HS * WS];
+  const IeeeDiv dvx{a.dx}, dvz{a.dz};
+
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  const int x0 = (tile % a.ntx) * TX, z0 = (tile / a.ntx) * TZ;
+  const int tid = threadIdx.x;
+  auto in_grid = [&](int x, int z) {
+    return x >= 0 && x < a.nx && z >= 0 && z < a.nz;
+  };
+  auto is_own = [&](int x, int z) {
+    return x >= x0 && x < x0 + TX && z >= z0 && z < z0 + TZ;
+  };
+
+  // phase A: every global load.  v on the 8-ring (inside the zero margin),
+  // stresses on the 6-ring (zero outside the grid) to LDS; the materials of
+  // this thread's items of phases B-E to registers.
+  for (int i = tid; i < HV * WV; i += NT) {
+    int r = i / WV, c = i - r * WV;
+    long long g = pidx(a, x0 - 8 + c, z0 - 8 + r);
+    s_vx[i] = a.vx_i[g];
+    s_vz[i] = a.vz_i[g];
+  }
+  double lmB[NB], muB[NB], bC[NC], lmD[ND], muD[ND], bE[NE];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    lmB[k] = muB[k] = 0.0;
+    if (i < HS * WS) {
+      int r = i / WS, c = i - r * WS;
+      int x = x0 - 6 + c, z = z0 - 6 + r;
+      double txx = 0.0, tzz = 0.0, txz = 0.0;
+      if (in_grid(x, z)) {
+        long long g = pidx(a, x, z);
+        txx = a.txx_i[g];
+        tzz = a.tzz_i[g];
+        txz = a.txz_i[g];
+        if (interior(a, x, z)) {
+          lmB[k] = a.lm[g];
+          muB[k] = a.mu[g];
+        }
+      }
+      s_txx[i] = txx;
+      s_tzz[i] = tzz;
+      s_txz[i] = txz;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    int i = tid + k * NT;
+    bC[k] = 0.0;
+    if (i < H4 * W4) {
+      int r = i / W4, c = i - r * W4;
+      int x = x0 - 4 + c, z = z0 - 4 + r;
+      if (in_grid(x, z)) bC[k] = a.b[pidx(a, x, z)];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    int i = tid + k * NT;
+    lmD[k] = muD[k] = 0.0;
+    if (i < H2 * W2) {
+      int r = i / W2, c = i - r * W2;
+      int x = x0 - 2 + c, z = z0 - 2 + r;
+      if (in_grid(x, z) && interior(a, x, z)) {
+        long long g = pidx(a, x, z);
+        lmD[k] = a.lm[g];
+        muD[k] = a.mu[g];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    int i = tid + k * NT;
+    bE[k] = 0.0;
+    if (i < TZ * TX) {
+      int r = i / TX, c = i - r * TX;
+      int x = x0 + c, z = z0 + r;
+      if (in_grid(x, z)) bE[k] = a.b[pidx(a, x, z)];
+    }
+  }
+  __syncthreads();
+
+  // phase B: stresses of step s0 on the 6-ring (in place).  Owners store
+  // the forward snapshot of step s0's stresses or correlate (adjoint).
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    if (i >= HS * WS) break;
+    int r = i / WS, c = i - r * WS;
+    int x = x0 - 6 + c, z = z0 - 6 + r;
+    if (!in_grid(x, z)) continue;
+    double txx = s_txx[i], tzz = s_tzz[i], txz = s_txz[i];
+    int lv = (r + 2) * WV + (c + 2);
+    if (a.do_snap && is_own(x, z)) {
+      if (!a.adjoint) {
+        long long ci = (long long)z * a.nx + x;
+        a.snap[2][ci] = txx;
+        a.snap[3][ci] = tzz;
+        a.snap[4][ci] = txz;
+      } else {
+        owner_snapshot(a, a.snap, x, z, s_vx[lv], s_vz[lv], txx, tzz, txz);
+      }
+    }
+    if (interior(a, x, z)) {
+      stress_point(s_vx, s_vz, lv, 1, WV, lmB[k], muB[k], taper_at(a, x, z),
+                   a.dts, dvx, dvz, txx, tzz, txz);
+      s_txx[i] = txx;
+      s_tzz[i] = tzz;
+      s_txz[i] = txz;
+    }
+  }
+  __syncthreads();
+
+  // phase C: velocities of step s0 on the 4-ring (in place), with step s0's
+  // injection terms for every point of the ring; owners record receivers
+  // (forward) before and after, and store the forward snapshot of v.
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    int i = tid + k * NT;
+    if (i >= H4 * W4) break;
+    int r = i / W4, c = i - r * W4;
+    int x = x0 - 4 + c, z = z0 - 4 + r;
+    if (!in_grid(x, z)) continue;
+    int lv = (r + 4) * WV + (c + 4);
+    double vx = s_vx[lv], vz = s_vz[lv];
+    bool own = is_own(x, z);
+    int local = (z - z0) * TX + (x - x0);
+    if (own && !a.adjoint) {
+      if (a.do_snap) {
+        long long ci = (long long)z * a.nx + x;
+        a.snap[0][ci] = vx;
+        a.snap[1][ci] = vz;
+      }
+      record_owned(a, tile, local, a.it, vx, vz);
+    }
+    double b = bC[k];
+    if (interior(a, x, z)) {
+      velocity_point(s_txx, s_tzz, s_txz, (r + 2) * WS + (c + 2), 1, WS, b,
+                     taper_at(a, x, z), a.dts, dvx, dvz, vx, vz);
+    }
+    apply_terms(a, a.halo_off, a.halo_entries, tile, i, a.it, b, vx, vz);
+    s_vx[lv] = vx;
+    s_vz[lv] = vz;
+    if (own && !a.adjoint) record_owned(a, tile, local, a.it1, vx, vz);
+  }
+  __syncthreads();
+
+  // phase D: stresses of step s1 on the 2-ring (in place); owners handle the
+  // step-s1 snapshot of stresses / correlation and store the new stresses.
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    int i = tid + k * NT;
+    if (i >= H2 * W2) break;
+    int r = i / W2, c = i - r * W2;
+    int x = x0 - 2 + c, z = z0 - 2 + r;
+    if (!in_grid(x, z)) continue;
+    int ls = (r + 4) * WS + (c + 4);
+    int lv = (r + 6) * WV + (c + 6);
+    double txx = s_txx[ls], tzz = s_tzz[ls], txz = s_txz[ls];
+    bool own = is_own(x, z);
+    if (own && a.do_snap1) {
+      if (!a.adjoint) {
+        long long ci = (long long)z * a.nx + x;
+        a.snap1[2][ci] = txx;
+        a.snap1[3][ci] = tzz;
+        a.snap1[4][ci] = txz;
+      } else {
+        owner_snapshot(a, a.snap1, x, z, s_vx[lv], s_vz[lv], txx, tzz, txz);
+      }
+    }
+    if (interior(a, x, z)) {
+      stress_point(s_vx, s_vz, lv, 1, WV, lmD[k], muD[k], taper_at(a, x, z),
+                   a.dts, dvx, dvz, txx, tzz, txz);
+      s_txx[ls] = txx;
+      s_tzz[ls] = tzz;
+      s_txz[ls] = txz;
+    }
+    if (own) {
+      long long g = pidx(a, x, z);
+      a.txx_o[g] = txx;
+      a.tzz_o[g] = tzz;
+      a.txz_o[g] = txz;
+    }
+  }
+  __syncthreads();
+
+  // phase E: velocities of step s1 on the tile, step s1's injection terms.
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    int i = tid + k * NT;
+    if (i >= TZ * TX) break;
+    int r = i / TX, c = i - r * TX;
+    int x = x0 + c, z = z0 + r;
+    if (!in_grid(x, z)) continue;
+    int lv = (r + 8) * WV + (c + 8);
+    double vx = s_vx[lv], vz = s_vz[lv];
+    if (a.do_snap1 && !a.adjoint) {
+      long long ci = (long long)z * a.nx + x;
+      a.snap1[0][ci] = vx;
+      a.snap1[1][ci] = vz;
+    }
+    double b = bE[k];
+    if (interior(a, x, z)) {
+      velocity_point(s_txx, s_tzz, s_txz, (r + 6) * WS + (c + 6), 1, WS, b,
+                     taper_at(a, x, z), a.dts, dvx, dvz, vx, vz);
+    }
+    apply_terms(a, a.tile_off, a.entries, tile, r * TX + c, a.it1, b, vx, vz);
+    long long g = pidx(a, x, z);
+    a.vx_o[g] = vx;
+    a.vz_o[g] = vz;
+  }
+}
+
+// Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
+// thread per trace.
+__global__ void receiver_scan_kernel(double *rx, double *rz, int nr, int nt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * nr) return;
+  double *p = (t < nr ? rx : rz) + (long long)(t % nr) * nt;
+  double acc = p[0];
+  for (int it = 1; it < nt; ++it) {
+    acc = acc + p[it];
+    p[it] = acc;
   }
 }
 
@@ -271,7 +545,7 @@ __global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
     long long g = pidx(a, x, z);
     double txx = a.txx_i[g], tzz = a.tzz_i[g], txz = a.txz_i[g];
     if (a.do_snap)
-      owner_snapshot(a, x, z, a.vx_i[g], a.vz_i[g], txx, tzz, txz);
+      owner_snapshot(a, a.snap, x, z, a.vx_i[g], a.vz_i[g], txx, tzz, txz);
     if (interior(a, x, z)) {
       stress_point(a.vx_i, a.vz_i, g, 1, a.pitch, a.lm[g], a.mu[g],
                    taper_at(a, x, z), a.dts, dv.x, dv.z, txx, tzz, txz);
@@ -532,7 +806,39 @@ inline unsigned blocks_for(long long n, int bs) {
 }  // namespace
 
 int step_tile_height(int variant) {
-  return (variant == kFusedTz16 || variant == kFusedTz16Recip) ? 16 : 8;
+  return (variant == kFusedTz16 || variant == kFusedTz16Recip ||
+          variant == kTwoStepTz16)
+             ? 16
+             : 8;
+}
+bool step_is_two_step(int variant) {
+  return variant == kTwoStepTz8 || variant == kTwoStepTz16;
+}
+int step_single_variant(int variant) {
+  if (variant == kTwoStepTz8) return kFusedTz8;
+  if (variant == kTwoStepTz16) return kFusedTz16;
+  return variant;
+}
+
+hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
+                            hipEvent_t e0, hipEvent_t e1) {
+  int nblocks = a.xcd_remap ? 8 * ((a.ntiles + 7) / 8) : a.ntiles;
+  dim3 g(nblocks);
+  if (variant == kTwoStepTz8)
+    hipExtLaunchKernelGGL((step_two<8, 256>), g, dim3(256), 0, s, e0, e1, 0, a);
+  else if (variant == kTwoStepTz16)
+    hipExtLaunchKernelGGL((step_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
+                                hipStream_t s) {
+  if (nr <= 0 || nt <= 1) return hipSuccess;
+  hipLaunchKernelGGL(receiver_scan_kernel, dim3((2 * nr + 63) / 64), dim3(64),
+                     0, s, rx, rz, nr, nt);
+  return hipGetLastError();
 }
 bool step_is_inplace(int variant) {
   return variant == kSplitTz8 || variant == kSplitTz8Recip;

@@ -115,6 +115,8 @@ class Model {
   long long *rec_idx_ = nullptr;
   int *fwd_off_ = nullptr, *adj_off_ = nullptr;
   InjEntry *fwd_entries_ = nullptr, *adj_entries_ = nullptr;
+  int *fwd_halo_off_ = nullptr, *adj_halo_off_ = nullptr;
+  InjEntry *fwd_halo_ = nullptr, *adj_halo_ = nullptr;
   double *misfit_buf_ = nullptr;  // partials then the result
   mutable double *staging_ = nullptr;
   mutable size_t staging_n_ = 0;
@@ -127,7 +129,9 @@ class Model {
   bool graph_mode_ = false;
   std::map<long long, hipGraphExec_t> graphs_;
   void clear_graphs();
-  void enqueue_steps(StepArgs &a, int i_shot, bool forward, bool store);
+  void enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
+                    std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
+                    bool dump);
 };
 
 }  // namespace psv

@@ -114,8 +114,8 @@ void Model::allocate() {
   nrec_ = (long long)cfg_.n_shots * cfg_.nr * cfg_.nt;
   // Padded stencil layout, large enough for any tile height in use.
   ntx_ = (int)((d_.nx + kTileX - 1) / kTileX);
-  pitch_ = (int)round_up(kHaloX + (long long)ntx_ * kTileX + 4, 64);
-  rows_ = kHaloZ + (int)round_up(d_.nz, 16) + 4;
+  pitch_ = (int)round_up(kHaloX + (long long)ntx_ * kTileX + 8, 64);
+  rows_ = kHaloZ + (int)round_up(d_.nz, 16) + 8;
   origin_ = (long long)kHaloZ * pitch_ + kHaloX;
   npad_ = (long long)rows_ * pitch_;
   for (int k = 0; k < 2; ++k)
@@ -221,6 +221,66 @@ void Model::build_injection_tables() {
   std::vector<InjEntry> adj_entries;
   bucket(aterms, adj_off, adj_entries, 0);
 
+  // Halo tables: every term whose point lies in a tile's 4-point halo
+  // region, local index over the (64 + 8) x (tz + 8) region, in term order.
+  auto halo_bucket = [&](const std::vector<Term> &terms, std::vector<int> &off,
+                         std::vector<InjEntry> &out, int base) {
+    std::vector<std::vector<InjEntry>> per(ntiles_);
+    const int w4 = kTileX + 8;
+    for (const Term &t : terms) {
+      int tx0 = std::max(0, (t.x - 67 + kTileX * 1024) / kTileX - 1024);
+      int tx1 = std::min(ntx_ - 1, (t.x + 4) / kTileX);
+      int tz0 = std::max(0, (t.z - tz_ - 3 + tz_ * 1024) / tz_ - 1024);
+      int tz1 = std::min(ntz_ - 1, (t.z + 4) / tz_);
+      for (int a = tz0; a <= tz1; ++a)
+        for (int b = tx0; b <= tx1; ++b) {
+          int x0 = b * kTileX, z0 = a * tz_;
+          if (t.x < x0 - 4 || t.x >= x0 + kTileX + 4 || t.z < z0 - 4 ||
+              t.z >= z0 + tz_ + 4)
+            continue;
+          InjEntry e;
+          e.local = (t.z - (z0 - 4)) * w4 + (t.x - (x0 - 4));
+          e.comp = t.comp;
+          e.src = t.src;
+          e.pad = 0;
+          e.coef = t.coef;
+          per[a * ntx_ + b].push_back(e);
+        }
+    }
+    off.assign(ntiles_ + 1, base);
+    for (int i = 0; i < ntiles_; ++i) {
+      off[i + 1] = off[i] + (int)per[i].size();
+      out.insert(out.end(), per[i].begin(), per[i].end());
+    }
+  };
+  std::vector<int> fwd_halo_off;
+  std::vector<InjEntry> fwd_halo;
+  for (int s = 0; s < cfg_.n_shots; ++s) {
+    std::vector<Term> terms;
+    for (int src : cfg_.which_source_to_fire_in_which_shot[s]) {
+      int x = d_.ix_src[src], z = d_.iz_src[src];
+      const double *M = &d_.moment[(size_t)src * 4];
+      double q01 = 0.25 * M[1], q10 = 0.25 * M[2];
+      terms.push_back({x - 1, z, 0, src, -M[0]});
+      terms.push_back({x, z, 0, src, M[0]});
+      terms.push_back({x, z - 1, 1, src, -M[3]});
+      terms.push_back({x, z, 1, src, M[3]});
+      terms.push_back({x - 1, z + 1, 0, src, q01});
+      terms.push_back({x, z + 1, 0, src, q01});
+      terms.push_back({x - 1, z - 1, 0, src, -q01});
+      terms.push_back({x, z - 1, 0, src, -q01});
+      terms.push_back({x + 1, z - 1, 1, src, q10});
+      terms.push_back({x + 1, z, 1, src, q10});
+      terms.push_back({x - 1, z - 1, 1, src, -q10});
+      terms.push_back({x - 1, z, 1, src, -q10});
+    }
+    halo_bucket(terms, off, fwd_halo, (int)fwd_halo.size());
+    fwd_halo_off.insert(fwd_halo_off.end(), off.begin(), off.end());
+  }
+  std::vector<int> adj_halo_off;
+  std::vector<InjEntry> adj_halo;
+  halo_bucket(aterms, adj_halo_off, adj_halo, 0);
+
   // (Re)upload; these small tables are reallocated when the tiling changes.
   auto put = [&](void *&dst, const void *src, size_t bytes) {
     if (dst) {
@@ -236,12 +296,19 @@ void Model::build_injection_tables() {
       hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
   };
   if (fwd_off.empty()) fwd_off.assign(ntiles_ + 1, 0);
+  if (fwd_halo_off.empty()) fwd_halo_off.assign(ntiles_ + 1, 0);
   put((void *&)fwd_off_, fwd_off.data(), fwd_off.size() * sizeof(int));
   put((void *&)fwd_entries_, fwd_entries.data(),
       fwd_entries.size() * sizeof(InjEntry));
   put((void *&)adj_off_, adj_off.data(), adj_off.size() * sizeof(int));
   put((void *&)adj_entries_, adj_entries.data(),
       adj_entries.size() * sizeof(InjEntry));
+  put((void *&)fwd_halo_off_, fwd_halo_off.data(),
+      fwd_halo_off.size() * sizeof(int));
+  put((void *&)fwd_halo_, fwd_halo.data(), fwd_halo.size() * sizeof(InjEntry));
+  put((void *&)adj_halo_off_, adj_halo_off.data(),
+      adj_halo_off.size() * sizeof(int));
+  put((void *&)adj_halo_, adj_halo.data(), adj_halo.size() * sizeof(InjEntry));
 }
 
 void Model::set_step_variant(int variant, int xcd_remap) {
@@ -357,19 +424,45 @@ StepArgs Model::base_args() const {
   return a;
 }
 
-// Enqueues one time loop: zero the state, then one step launch per time step
-// (two for the split variants).  No synchronisation.
-void Model::enqueue_steps(StepArgs &a, int i_shot, bool forward, bool store) {
+// Enqueues one time loop: zero the state, then one launch per time step
+// (two for the split variants) or per pair of steps (two-step variants; an
+// odd last step uses the matching one-step kernel).  With `pairs`, every
+// timing_stride-th launch records its kernel start and end; with `dump`, the
+// reference's text snapshot is written every 10 steps (one-step launches).
+void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
+                         std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
+                         bool dump) {
   const bool inplace = step_is_inplace(variant_);
+  const bool two = step_is_two_step(variant_) && !dump;
+  const int single = two ? step_single_variant(variant_)
+                         : (dump ? step_single_variant(variant_) : variant_);
   const int nt = cfg_.nt, si = cfg_.snapshot_interval;
   // zero the state (src/fdModel.cpp:530-542, 792-805) with a fill kernel,
   // which is safe inside a captured graph
   for (int k = 0; k < (inplace ? 1 : 2); ++k)
     for (int f = 0; f < 5; ++f)
       hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
-  for (int k = 0; k < nt; ++k) {
-    const int it = forward ? k : nt - 1 - k;
-    const int cur = inplace ? 0 : (k & 1), nxt = inplace ? 0 : ((k + 1) & 1);
+  a.rec_increments = (forward && step_is_two_step(variant_)) ? 1 : 0;
+  a.rec_off = adj_off_;
+  a.rec_entries = adj_entries_;
+  size_t next_event = 0;
+  auto event = [&]() {
+    if (next_event == events_.size()) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      events_.push_back(e);
+    }
+    return events_[next_event++];
+  };
+  auto slot = [&](int it, double **dst) {
+    long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
+    for (int f = 0; f < 5; ++f) dst[f] = snap_[f] + off;
+  };
+  int launch = 0;
+  for (int k = 0; k < nt;) {
+    const int pair = two && k + 1 < nt;
+    const int cur = inplace ? 0 : (launch & 1);
+    const int nxt = inplace ? 0 : ((launch + 1) & 1);
     a.vx_i = state_[cur][0];
     a.vz_i = state_[cur][1];
     a.txx_i = state_[cur][2];
@@ -380,19 +473,61 @@ void Model::enqueue_steps(StepArgs &a, int i_shot, bool forward, bool store) {
     a.txx_o = state_[nxt][2];
     a.tzz_o = state_[nxt][3];
     a.txz_o = state_[nxt][4];
+    const int it = forward ? k : nt - 1 - k;
+    const int it1 = forward ? k + 1 : nt - 2 - k;
     a.it = it;
     a.do_snap = (it % si == 0) && (store || !forward);
-    if (a.do_snap) {
-      long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
-      for (int f = 0; f < 5; ++f) a.snap[f] = snap_[f] + off;
+    if (a.do_snap) slot(it, a.snap);
+    a.it1 = it1;
+    a.do_snap1 = pair && (it1 % si == 0) && (store || !forward);
+    if (a.do_snap1) slot(it1, a.snap1);
+    const int covered = pair ? 2 : 1;
+    bool timed = pairs && timing_stride_ > 0 && (launches_ / covered) % timing_stride_ == 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+      e0 = event();
+      e1 = event();
+      pairs->emplace_back(e0, e1);
     }
-    hip_check(launch_step(variant_, a, stream_), "step kernel");
+    if (pair)
+      hip_check(launch_two_steps(variant_, a, stream_, e0, e1), "two-step kernel");
+    else
+      hip_check(launch_step(single, a, stream_, e0, e1), "step kernel");
+    launches_ += covered;
+    if (timed) timed_ += covered;
+    if (dump && it % 10 == 0) {
+      // src/fdModel.cpp:753-776: state after step it, default stream format
+      std::vector<double> hx(ngrid_), hz(ngrid_);
+      double *s = staging(ngrid_);
+      hip_check(launch_unpad(state_[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
+                             stream_),
+                "unpad");
+      to_host_layout(hx.data(), s);
+      hip_check(launch_unpad(state_[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
+                             stream_),
+                "unpad");
+      to_host_layout(hz.data(), s);
+      std::ofstream fx("snapshots/vx" + zero_pad(it, 5) + ".txt");
+      std::ofstream fz("snapshots/vz" + zero_pad(it, 5) + ".txt");
+      for (int ix = 0; ix < d_.nx; ++ix) {
+        for (int iz = 0; iz < d_.nz; ++iz) {
+          fx << hx[(size_t)ix * d_.nz + iz] << " ";
+          fz << hz[(size_t)ix * d_.nz + iz] << " ";
+        }
+        fx << std::endl;
+        fz << std::endl;
+      }
+    }
+    k += covered;
+    ++launch;
   }
+  if (a.rec_increments)
+    hip_check(launch_receiver_scan(a.rtf_x, a.rtf_z, cfg_.nr, nt, stream_),
+              "receiver scan");
 }
 
-// The time loop.  Plain mode launches step by step (with optional HIP-event
-// brackets around every timing_stride-th launch, and the reference's text
-// dump); graph mode replays a captured graph of the whole loop.
+// The time loop.  Plain mode launches step by step; graph mode replays a
+// captured graph of the whole loop (with timing, bracketing the replay).
 void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
                       bool dump) {
   const int nt = cfg_.nt;
@@ -404,14 +539,16 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
       hipGraph_t graph = nullptr;
       hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
                 "hipStreamBeginCapture");
+      int64_t saved = launches_;
       try {
-        enqueue_steps(a, i_shot, forward, store);
+        enqueue_loop(a, i_shot, forward, store, nullptr, false);
       } catch (...) {
         hipGraph_t g2 = nullptr;
         (void)hipStreamEndCapture(stream_, &g2);
         if (g2) (void)hipGraphDestroy(g2);
         throw;
       }
+      launches_ = saved;
       hip_check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
       hipGraphExec_t exec = nullptr;
       hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -443,80 +580,14 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
     }
     return;
   }
-  const bool inplace = step_is_inplace(variant_);
-  const int si = cfg_.snapshot_interval;
-  for (int k = 0; k < (inplace ? 1 : 2); ++k)
-    for (int f = 0; f < 5; ++f)
-      hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
-  size_t next_event = 0;
-  auto event = [&]() {
-    if (next_event == events_.size()) {
-      hipEvent_t e;
-      hip_check(hipEventCreate(&e), "hipEventCreate");
-      events_.push_back(e);
-    }
-    return events_[next_event++];
-  };
-  for (int k = 0; k < nt; ++k) {
-    const int it = forward ? k : nt - 1 - k;
-    const int cur = inplace ? 0 : (k & 1), nxt = inplace ? 0 : ((k + 1) & 1);
-    a.vx_i = state_[cur][0];
-    a.vz_i = state_[cur][1];
-    a.txx_i = state_[cur][2];
-    a.tzz_i = state_[cur][3];
-    a.txz_i = state_[cur][4];
-    a.vx_o = state_[nxt][0];
-    a.vz_o = state_[nxt][1];
-    a.txx_o = state_[nxt][2];
-    a.tzz_o = state_[nxt][3];
-    a.txz_o = state_[nxt][4];
-    a.it = it;
-    a.do_snap = (it % si == 0) && (store || !forward);
-    if (a.do_snap) {
-      long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
-      for (int f = 0; f < 5; ++f) a.snap[f] = snap_[f] + off;
-    }
-    bool timed = timing_stride_ > 0 && (launches_ % timing_stride_) == 0;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (timed) {
-      e0 = event();
-      e1 = event();
-      pairs.emplace_back(e0, e1);
-    }
-    hip_check(launch_step(variant_, a, stream_, e0, e1), "step kernel");
-    ++launches_;
-    if (dump && it % 10 == 0) {
-      // src/fdModel.cpp:753-776: state after step it, default stream format
-      std::vector<double> hx(ngrid_), hz(ngrid_);
-      double *s = staging(ngrid_);
-      hip_check(launch_unpad(state_[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
-                             stream_),
-                "unpad");
-      to_host_layout(hx.data(), s);
-      hip_check(launch_unpad(state_[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
-                             stream_),
-                "unpad");
-      to_host_layout(hz.data(), s);
-      std::ofstream fx("snapshots/vx" + zero_pad(it, 5) + ".txt");
-      std::ofstream fz("snapshots/vz" + zero_pad(it, 5) + ".txt");
-      for (int ix = 0; ix < d_.nx; ++ix) {
-        for (int iz = 0; iz < d_.nz; ++iz) {
-          fx << hx[(size_t)ix * d_.nz + iz] << " ";
-          fz << hz[(size_t)ix * d_.nz + iz] << " ";
-        }
-        fx << std::endl;
-        fz << std::endl;
-      }
-    }
-  }
+  enqueue_loop(a, i_shot, forward, store, &pairs, dump);
   sync();
   for (auto &p : pairs) {
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, p.first, p.second),
               "hipEventElapsedTime");
     timed_ms_ += ms;
-    ++timed_;
   }
 }
 
@@ -532,6 +603,8 @@ void Model::forward(int i_shot, bool store_fields, bool verbose,
   a.dts = cfg_.dt;
   a.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
   a.entries = fwd_entries_;
+  a.halo_off = fwd_halo_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.halo_entries = fwd_halo_;
   a.rtf_x = syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   a.rtf_z = syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   auto t0 = std::chrono::steady_clock::now();
@@ -557,6 +630,8 @@ void Model::adjoint(int i_shot, bool verbose) {
   a.dts = -cfg_.dt;
   a.tile_off = adj_off_;
   a.entries = adj_entries_;
+  a.halo_off = adj_halo_off_;
+  a.halo_entries = adj_halo_;
   a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   auto t0 = std::chrono::steady_clock::now();

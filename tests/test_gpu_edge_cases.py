@@ -68,7 +68,7 @@ def test_sources_and_receivers_on_tile_boundaries(oracle_lib):
                           moment_angles=[30, 60, 90])
     cfg["receivers"].update(ix_receivers=[52, 51, 52, 52, 63, 0, 89],
                             iz_receivers=[4, 4, 3, 4, 11, 0, 49])
-    for variant in range(6):
+    for variant in range(8):
         p, o = _run_both(oracle_lib, cfg, variant)
         _assert_same(p, o)
 
@@ -82,8 +82,9 @@ def test_coincident_sources_and_receivers_keep_reference_order(oracle_lib):
                           n_shots=2)
     cfg["receivers"].update(nr=5, ix_receivers=[40, 40, 40, 3, 40],
                             iz_receivers=[30, 30, 30, 4, 31])
-    p, o = _run_both(oracle_lib, cfg)
-    _assert_same(p, o)
+    for variant in (0, 6, 7):
+        p, o = _run_both(oracle_lib, cfg, variant)
+        _assert_same(p, o)
 
 
 @pytest.mark.parametrize("nx_inner,nz_inner,np_b", [(37, 13, 0), (3, 3, 2), (130, 21, 7)])
@@ -99,7 +100,7 @@ def test_odd_grid_sizes(oracle_lib, nx_inner, nz_inner, np_b):
                             iz_receivers=[nz_inner - 1, 0])
     if np_b == 0:  # a source needs one point of margin around it
         cfg["sources"]["ix_sources"] = [max(1, nx_inner // 2)]
-    for variant in (0, 2, 3):
+    for variant in (0, 2, 3, 6, 7):
         p, o = _run_both(oracle_lib, cfg, variant)
         _assert_same(p, o)
 
@@ -111,7 +112,7 @@ def test_variants_and_xcd_mapping_agree_bitwise(oracle_lib):
     cfg["receivers"]["ix_receivers"] = [5, 17, 130, 151, 152, 275, 288]
     p, o = _run_both(oracle_lib, cfg, 0, True)
     _assert_same(p, o)
-    for variant in range(6):
+    for variant in range(8):
         for remap in (True, False):
             _assert_same(_run_product_only(cfg, variant, remap), o)
 
@@ -136,7 +137,7 @@ def _run_product_only(cfg, variant, remap):
 
 def test_basis_functions_and_kernel_accumulation(oracle_lib):
     cfg = small_config(**{"domain.nt": 100, "basis.npx": 4, "basis.npz": 2})
-    m = _product(cfg)
+    m = _product(cfg, 6)
     o = oracle_lib.OracleModel(cfg)
     for mod in (m, o):
         mod.set_parameter_fields(*_random_fields(mod.nx, mod.nz, 3))
@@ -248,9 +249,9 @@ def test_graph_replays_match_eager_mode(oracle_lib):
     """Each time loop captured once and replayed: every replay, with eager
     work (parameter changes, misfit, reset) between replays, must equal the
     eager result bit for bit."""
-    cfg = small_config(**{"domain.nt": 120})
+    cfg = small_config(**{"domain.nt": 121})
     eager = _product(cfg, 0)
-    graph = _product(cfg, 0)
+    graph = _product(cfg, 6)
     graph.set_graph_mode(True)
     for rep in range(3):
         fields = _random_fields(eager.nx, eager.nz, rep)

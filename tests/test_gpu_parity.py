@@ -114,11 +114,26 @@ def _small_models(cfg):
     return true, start
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", list(range(8)))
 def test_small_chain_matches_oracle(oracle_lib, variant):
     cfg = small_config()
     true, start = _small_models(cfg)
     print(f"[parity] small chain, variant {variant}", flush=True)
+    p = _chain(_product(cfg, variant), true, start)
+    o, _ = _oracle_chain(oracle_lib, cfg, true, start)
+    _compare(p, o)
+
+
+@pytest.mark.parametrize("variant", [6, 7])
+def test_two_step_odd_nt_matches_oracle(oracle_lib, variant):
+    """Two-step launches with an odd number of steps (the last step runs on
+    the one-step kernel) and snapshot steps at odd time indices."""
+    cfg = small_config(**{"domain.nt": 151, "inversion.snapshot_interval": 151})
+    true, start = _small_models(cfg)
+    p = _chain(_product(cfg, variant), true, start)
+    o, _ = _oracle_chain(oracle_lib, cfg, true, start)
+    _compare(p, o)
+    cfg = small_config(**{"domain.nt": 147, "inversion.snapshot_interval": 7})
     p = _chain(_product(cfg, variant), true, start)
     o, _ = _oracle_chain(oracle_lib, cfg, true, start)
     _compare(p, o)
