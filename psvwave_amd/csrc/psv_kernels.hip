@@ -169,39 +169,42 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
   const int tid = threadIdx.x;
 
-  // phase A: issue every global load of the block up front.  Velocities
-  // with a 4-point halo (inside the zero margin) and stresses on the ring go
-  // to LDS; materials of this thread's stress and velocity items go to
-  // registers.  Points outside the grid stage zeros.
-  for (int i = tid; i < HV * WV; i += NT) {
-    int r = i / WV, c = i - r * WV;
-    long long g = pidx(a, x0 - 4 + c, z0 - 4 + r);
-    s_vx[i] = a.vx_i[g];
-    s_vz[i] = a.vz_i[g];
+  // phase A: issue every global load of the block before the first wait.
+  // Velocities with a 4-point halo (inside the zero margin) and stresses on
+  // the ring are loaded into registers by fully unrolled loops, then written
+  // to LDS; the materials of this thread's items stay in registers.  Points
+  // outside the grid stage zeros.
+  constexpr int NV = (HV * WV + NT - 1) / NT;
+  double vx_r[NV], vz_r[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int i = tid + k * NT;
+    vx_r[k] = vz_r[k] = 0.0;
+    if (i < HV * WV) {
+      int r = i / WV, c = i - r * WV;
+      long long g = pidx(a, x0 - 4 + c, z0 - 4 + r);
+      vx_r[k] = a.vx_i[g];
+      vz_r[k] = a.vz_i[g];
+    }
   }
-  double lm_r[NB], mu_r[NB], b_r[NC];
+  double txx_r[NB], tzz_r[NB], txz_r[NB], lm_r[NB], mu_r[NB], b_r[NC];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     int i = tid + k * NT;
-    lm_r[k] = 0.0;
-    mu_r[k] = 0.0;
+    txx_r[k] = tzz_r[k] = txz_r[k] = lm_r[k] = mu_r[k] = 0.0;
     if (i < HS * WS) {
       int r = i / WS, c = i - r * WS;
       int x = x0 - 2 + c, z = z0 - 2 + r;
-      double txx = 0.0, tzz = 0.0, txz = 0.0;
       if (x >= 0 && x < a.nx && z >= 0 && z < a.nz) {
         long long g = pidx(a, x, z);
-        txx = a.txx_i[g];
-        tzz = a.tzz_i[g];
-        txz = a.txz_i[g];
+        txx_r[k] = a.txx_i[g];
+        tzz_r[k] = a.tzz_i[g];
+        txz_r[k] = a.txz_i[g];
         if (interior(a, x, z)) {
           lm_r[k] = a.lm[g];
           mu_r[k] = a.mu[g];
         }
       }
-      s_txx[i] = txx;
-      s_tzz[i] = tzz;
-      s_txz[i] = txz;
     }
   }
 #pragma unroll
@@ -212,6 +215,23 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
       int r = i / kTileX, c = i - r * kTileX;
       int x = x0 + c, z = z0 + r;
       if (x < a.nx && z < a.nz) b_r[k] = a.b[pidx(a, x, z)];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int i = tid + k * NT;
+    if (i < HV * WV) {
+      s_vx[i] = vx_r[k];
+      s_vz[i] = vz_r[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    if (i < HS * WS) {
+      s_txx[i] = txx_r[k];
+      s_tzz[i] = tzz_r[k];
+      s_txz[i] = txz_r[k];
     }
   }
   if (!a.adjoint && tile == 0) record_receivers(a);
@@ -226,7 +246,7 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
     int r = i / WS, c = i - r * WS;
     int x = x0 - 2 + c, z = z0 - 2 + r;
     if (x < 0 || x >= a.nx || z < 0 || z >= a.nz) continue;
-    double txx = s_txx[i], tzz = s_tzz[i], txz = s_txz[i];
+    double txx = txx_r[k], tzz = tzz_r[k], txz = txz_r[k];
     bool own = r >= 2 && r < TZ + 2 && c >= 2 && c < kTileX + 2;
     if (own && a.do_snap && !a.adjoint) {
       long long ci = (long long)z * a.nx + x;
@@ -316,37 +336,42 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
     return x >= x0 && x < x0 + TX && z >= z0 && z < z0 + TZ;
   };
 
-  // phase A: every global load.  v on the 8-ring (inside the zero margin),
-  // stresses on the 6-ring (zero outside the grid) to LDS; the materials of
-  // this thread's items of phases B-E to registers.
-  for (int i = tid; i < HV * WV; i += NT) {
-    int r = i / WV, c = i - r * WV;
-    long long g = pidx(a, x0 - 8 + c, z0 - 8 + r);
-    s_vx[i] = a.vx_i[g];
-    s_vz[i] = a.vz_i[g];
+  // phase A: every global load before the first wait (fully unrolled loops
+  // into registers, then LDS).  v on the 8-ring (inside the zero margin),
+  // stresses on the 6-ring (zero outside the grid); the materials of this
+  // thread's items of phases B-E stay in registers.
+  constexpr int NV = (HV * WV + NT - 1) / NT;
+  double vx_r[NV], vz_r[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int i = tid + k * NT;
+    vx_r[k] = vz_r[k] = 0.0;
+    if (i < HV * WV) {
+      int r = i / WV, c = i - r * WV;
+      long long g = pidx(a, x0 - 8 + c, z0 - 8 + r);
+      vx_r[k] = a.vx_i[g];
+      vz_r[k] = a.vz_i[g];
+    }
   }
+  double txx_r[NB], tzz_r[NB], txz_r[NB];
   double lmB[NB], muB[NB], bC[NC], lmD[ND], muD[ND], bE[NE];
 #pragma unroll
   for (int k = 0; k < NB; ++k) {
     int i = tid + k * NT;
-    lmB[k] = muB[k] = 0.0;
+    txx_r[k] = tzz_r[k] = txz_r[k] = lmB[k] = muB[k] = 0.0;
     if (i < HS * WS) {
       int r = i / WS, c = i - r * WS;
       int x = x0 - 6 + c, z = z0 - 6 + r;
-      double txx = 0.0, tzz = 0.0, txz = 0.0;
       if (in_grid(x, z)) {
         long long g = pidx(a, x, z);
-        txx = a.txx_i[g];
-        tzz = a.tzz_i[g];
-        txz = a.txz_i[g];
+        txx_r[k] = a.txx_i[g];
+        tzz_r[k] = a.tzz_i[g];
+        txz_r[k] = a.txz_i[g];
         if (interior(a, x, z)) {
           lmB[k] = a.lm[g];
           muB[k] = a.mu[g];
         }
       }
-      s_txx[i] = txx;
-      s_tzz[i] = tzz;
-      s_txz[i] = txz;
     }
   }
 #pragma unroll
@@ -383,6 +408,23 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
       if (in_grid(x, z)) bE[k] = a.b[pidx(a, x, z)];
     }
   }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int i = tid + k * NT;
+    if (i < HV * WV) {
+      s_vx[i] = vx_r[k];
+      s_vz[i] = vz_r[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    int i = tid + k * NT;
+    if (i < HS * WS) {
+      s_txx[i] = txx_r[k];
+      s_tzz[i] = tzz_r[k];
+      s_txz[i] = txz_r[k];
+    }
+  }
   __syncthreads();
 
   // phase B: stresses of step s0 on the 6-ring (in place).  Owners store
@@ -394,7 +436,7 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
     int r = i / WS, c = i - r * WS;
     int x = x0 - 6 + c, z = z0 - 6 + r;
     if (!in_grid(x, z)) continue;
-    double txx = s_txx[i], tzz = s_tzz[i], txz = s_txz[i];
+    double txx = txx_r[k], tzz = tzz_r[k], txz = txz_r[k];
     int lv = (r + 2) * WV + (c + 2);
     if (a.do_snap && is_own(x, z)) {
       if (!a.adjoint) {
