@@ -154,8 +154,9 @@ struct Divs<true> {
   __device__ explicit Divs(const StepArgs &a) : x{a.dx, a.rdx}, z{a.dz, a.rdz} {}
 };
 
-template <int TZ, bool RECIP, int NT = kBlock>
-__global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
+// One time step of one 64 x TZ tile (the fused one-step kernel's body).
+template <int TZ, bool RECIP, int NT>
+__device__ __forceinline__ void fused_tile(const StepArgs &a, const int tile) {
   constexpr int WV = kTileX + 8, HV = TZ + 8;  // velocity halo region
   constexpr int WS = kTileX + 4, HS = TZ + 4;  // stress ring region
   constexpr int NB = (HS * WS + NT - 1) / NT;  // stress items per thread
@@ -163,9 +164,6 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
   __shared__ double s_vx[HV * WV], s_vz[HV * WV];
   __shared__ double s_txx[HS * WS], s_tzz[HS * WS], s_txz[HS * WS];
   const Divs<RECIP> dv(a);
-
-  const int tile = block_tile(a);
-  if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
   const int tid = threadIdx.x;
 
@@ -303,6 +301,13 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
 }
 
 
+template <int TZ, bool RECIP, int NT = kBlock>
+__global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  fused_tile<TZ, RECIP, NT>(a, tile);
+}
+
 // Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
 // velocities with an 8-point halo and stresses with a 6-point ring, then
 // updates in place in LDS: stresses of step s0 on the 6-ring, velocities on
@@ -311,7 +316,7 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
 // step s1 on the tile.  Global traffic per point-update is about half that
 // of a one-step launch; the stencil work on the rings is recomputed.
 template <int TZ, int NT>
-__global__ void __launch_bounds__(NT) step_two(StepArgs a) {
+__device__ __forceinline__ void two_step_tile(const StepArgs &a, const int tile) {
   constexpr int TX = kTileX;
   constexpr int WV = TX + 16, HV = TZ + 16;  // v on the 8-ring
   constexpr int WS = TX + 12, HS = TZ + 12;  // stresses on the 6-ring
@@ -324,9 +329,6 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
   __shared__ double s_vx[HV * WV], s_vz[HV * WV];
   __shared__ double s_txx[HS * WS], s_tzz[HS * WS], s_txz[HS * WS];
   const IeeeDiv dvx{a.dx}, dvz{a.dz};
-
-  const int tile = block_tile(a);
-  if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * TX, z0 = (tile / a.ntx) * TZ;
   const int tid = threadIdx.x;
   auto in_grid = [&](int x, int z) {
@@ -556,6 +558,13 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
     a.vx_o[g] = vx;
     a.vz_o[g] = vz;
   }
+}
+
+template <int TZ, int NT>
+__global__ void __launch_bounds__(NT) step_two(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  two_step_tile<TZ, NT>(a, tile);
 }
 
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
