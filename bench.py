@@ -53,6 +53,10 @@ def _args():
                    help="first run a plain copy kernel over BYTES (the "
                         "reference point for byte counters) and report it")
     p.add_argument("--only-calibrate", action="store_true")
+    p.add_argument("--workload", choices=["single", "mc"], default="single",
+                   help="single: the layered-faulted model (default); mc: the "
+                        "512-model Monte-Carlo forward batch")
+    p.add_argument("--mc-models", type=int, default=512)
     return p.parse_args()
 
 
@@ -102,6 +106,73 @@ def cpu_baseline(nt_sample):
                        f"OpenMP {threads} threads)")}
 
 
+def monte_carlo(a):
+    """Forward-only throughput of the Monte-Carlo batch: 512 Gaussian
+    random-field models of uneven size (87M grid points in total), 2 shots,
+    nt = 3000; one step = forward of both shots for every model + misfits."""
+    import psvwave_amd
+    from psvwave_amd.synthetic import monte_carlo_configs, monte_carlo_model
+    cfgs = monte_carlo_configs(n_models=a.mc_models)
+    t0 = time.perf_counter()
+    batch = psvwave_amd.Batch(cfgs)
+    rng = np.random.default_rng(3)
+    # observed data: the homogeneous background of each member (setup)
+    for m, c in enumerate(cfgs):
+        i = batch.infos[m]
+        shape = (i["nx"], i["nz"])
+        med = c["medium"]
+        batch.set_parameter_fields(m, np.full(shape, med["scalar_vp"]),
+                                   np.full(shape, med["scalar_vs"]),
+                                   np.full(shape, med["scalar_rho"]))
+    best, tuned = batch.autotune(steps=20)
+    if a.graph:
+        batch.set_graph_mode(True)
+    batch.forward_all()
+    for m in range(len(cfgs)):
+        batch.set_observed_data(m, *batch.get_synthetic_data(m))
+        batch.set_parameter_fields(m, *monte_carlo_model(cfgs[m], rng))
+    setup = time.perf_counter() - t0
+
+    def step():
+        batch.forward_all()
+        return batch.misfits()
+
+    for _ in range(a.warmup):
+        step()
+    batch.timing(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        mis = step()
+    secs = time.perf_counter() - t0
+    loop_ms, points = batch.timing()
+    nt = batch.nt
+    upd = 2.0 * points * nt  # two shots
+    value = upd * a.steps / secs / 1e9
+    loop_s_per_step = loop_ms / 1e3 / a.steps
+    achieved = 104.0 * upd / loop_s_per_step / 1e9
+    return {
+        "metric": METRIC + " (Monte-Carlo forward batch)",
+        "value": round(value, 3), "unit": "Gpt-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(secs / a.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian random fields, observed data from "
+                "each member's homogeneous background)",
+        "config": {"workload": "Monte-Carlo forward batch", "models": len(cfgs),
+                   "grid_points": points, "nt": nt, "shots": 2, "receivers": 32,
+                   "point_updates_per_step": upd, "variant": best,
+                   "autotune_ms_per_step": tuned, "hip_graph": a.graph,
+                   "setup_s": round(setup, 1)},
+        "roofline": {"bound": "hbm", "kernel": f"batch step (variant {best})",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "note": "time = whole forward loops (HIP events), gaps included"},
+        "misfit_mean": float(np.mean(mis)),
+    }
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -117,6 +188,9 @@ def main():
         print(json.dumps({"copy_calibration": copy}), file=sys.stderr, flush=True)
         if a.only_calibrate:
             return
+    if a.workload == "mc":
+        print(json.dumps(monte_carlo(a)), flush=True)
+        return
     import psvwave_amd
     from psvwave_amd.synthetic import (algorithmic_bytes_per_shot,
                                        layered_faulted_config,

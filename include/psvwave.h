@@ -204,6 +204,41 @@ int psv_device_bytes(const psv_model *m, int64_t *bytes);
  * Reference point for byte counters and sustainable bandwidth. */
 int psv_calibrate_copy(int64_t bytes, int iters, double *ms);
 
+/* ---- Batched Monte-Carlo forward runs (no reference counterpart: the
+ * reference runs one fdModel at a time, README.md:16-17 states the
+ * Monte-Carlo purpose; copy() at src/psvWave.cpp:421-425 is its only way to
+ * multiply models).  A batch holds n independent models of possibly
+ * different sizes sharing nt and n_shots; psv_batch_forward advances every
+ * model of one shot with one launch per time step over all their tiles.
+ * Each member behaves as the model psv_create(&params[m]) would:
+ * seismograms are bit-identical to running the members one by one. */
+typedef struct psv_batch psv_batch;
+int psv_batch_create(int n, const psv_params *params, psv_batch **out);
+void psv_batch_destroy(psv_batch *b);
+int psv_batch_size(const psv_batch *b, int *n);
+int psv_batch_get_info(const psv_batch *b, int m, psv_info *out);
+int psv_batch_set_parameter_fields(psv_batch *b, int m, const double *vp,
+                                   const double *vs, const double *rho);
+int psv_batch_set_observed_data(psv_batch *b, int m, const double *ux,
+                                const double *uz);
+int psv_batch_get_synthetic_data(const psv_batch *b, int m, double *ux,
+                                 double *uz);
+/* Forward simulation of shot i_shot for every model (no snapshots). */
+int psv_batch_forward(psv_batch *b, int i_shot);
+/* L2 misfit of every model (n values), as calculate_l2_misfit. */
+int psv_batch_misfits(psv_batch *b, double *out);
+/* Step kernel for all members: 0, 1, 6 or 7 (see psv_set_step_variant). */
+int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
+int psv_batch_set_graph_mode(psv_batch *b, int on);
+/* Times `steps` batched steps of variants 0, 1, 6, 7 (ms_per_step[4]) and
+ * keeps the fastest (*best).  Overwrites shot 0's synthetic data. */
+int psv_batch_autotune(psv_batch *b, int steps, int *best,
+                       double *ms_per_step);
+/* Summed wall time of psv_batch_forward time loops (HIP events) and the
+ * number of grid points of all members. */
+int psv_batch_timing(psv_batch *b, double *loop_ms, int64_t *points,
+                     int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,9 +112,22 @@ _SIGNATURES = {
     "psv_reset_step_timing": [_P],
     "psv_device_bytes": [_P, C.POINTER(C.c_int64)],
     "psv_calibrate_copy": [C.c_int64, _i, _D],
+    "psv_batch_create": [_i, C.POINTER(Params), C.POINTER(_P)],
+    "psv_batch_destroy": [_P],
+    "psv_batch_size": [_P, C.POINTER(C.c_int)],
+    "psv_batch_get_info": [_P, _i, C.POINTER(Info)],
+    "psv_batch_set_parameter_fields": [_P, _i, _D, _D, _D],
+    "psv_batch_set_observed_data": [_P, _i, _D, _D],
+    "psv_batch_get_synthetic_data": [_P, _i, _D, _D],
+    "psv_batch_forward": [_P, _i],
+    "psv_batch_misfits": [_P, _D],
+    "psv_batch_set_variant": [_P, _i, _i],
+    "psv_batch_set_graph_mode": [_P, _i],
+    "psv_batch_autotune": [_P, _i, C.POINTER(C.c_int), _D],
+    "psv_batch_timing": [_P, _D, C.POINTER(C.c_int64), _i],
 }
 _RESTYPES = {"psv_last_error": C.c_char_p, "psv_config_destroy": None,
-             "psv_destroy": None}
+             "psv_destroy": None, "psv_batch_destroy": None}
 
 _lib = None
 

@@ -1,0 +1,59 @@
+// psv_batch.h -- many independent fdModels advanced together: one launch per
+// time step covers every tile of every model (Monte-Carlo forward runs over
+// models of uneven size).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "psv_model.h"
+
+namespace psv {
+
+class Batch {
+ public:
+  // All members must share nt and n_shots.
+  explicit Batch(const std::vector<Config> &cfgs);
+  ~Batch();
+  Batch(const Batch &) = delete;
+  Batch &operator=(const Batch &) = delete;
+
+  int size() const { return (int)models_.size(); }
+  Model &model(int m);
+  const Model &model(int m) const;
+
+  // Step kernel for every member: 0 / 1 one-step 64x8 / 64x16 tiles,
+  // 6 / 7 two-step 64x8 / 64x16 tiles.
+  void set_variant(int variant, int xcd_remap);
+  int variant() const { return variant_; }
+  void set_graph_mode(bool on);
+  // Forward simulation of shot i_shot for every member (no snapshots).
+  void forward(int i_shot);
+  // Per-member L2 misfit over all shots (calculate_l2_misfit of each).
+  void misfits(double *out);
+  // Times `steps` batched steps of each candidate variant, keeps the
+  // fastest; returns its time per step in ms.
+  double autotune(int steps, int *best, double *ms_per_variant);
+  long long grid_points() const { return points_; }
+  double loop_ms() const { return loop_ms_; }
+  void reset_timing() { loop_ms_ = 0.0; }
+
+ private:
+  void build_tables();
+  void enqueue_loop(int i_shot, int nsteps);
+  void clear_graphs();
+
+  std::vector<std::unique_ptr<Model>> models_;
+  int nt_ = 0, n_shots_ = 0, variant_ = kFusedTz8, xcd_remap_ = 1;
+  hipStream_t stream_ = nullptr;
+  StepArgs *descs_ = nullptr;  // [model][shot][parity]
+  int2 *tiles_ = nullptr;      // (model, tile) pairs
+  int ntiles_ = 0;
+  long long points_ = 0;
+  bool graph_mode_ = false;
+  std::vector<hipGraphExec_t> graphs_;  // per shot
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  double loop_ms_ = 0.0;
+};
+
+}  // namespace psv

@@ -1,0 +1,200 @@
+// psv_batch.hip -- batched forward runs over many models of uneven size.
+//
+// Each member is a full Model (its own HBM buffers and injection tables).
+// The batch keeps, in device memory, the launch arguments of every
+// (member, shot, buffer parity) and the list of (member, tile) pairs of all
+// members, so one launch per time step advances every model: the grid is
+// the union of all members' tiles, large enough to fill the GPU even when
+// each model alone is small.  A shot's whole time loop can be captured as
+// one HIP graph and replayed.
+#include "psv_batch.h"
+
+#include <algorithm>
+#include <limits>
+
+namespace psv {
+
+namespace {
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw RuntimeError(std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace
+
+Batch::Batch(const std::vector<Config> &cfgs) {
+  if (cfgs.empty()) throw ValueError("a batch needs at least one model");
+  nt_ = cfgs[0].nt;
+  n_shots_ = cfgs[0].n_shots;
+  for (const Config &c : cfgs)
+    if (c.nt != nt_ || c.n_shots != n_shots_)
+      throw ValueError("all models of a batch must share nt and n_shots");
+  for (const Config &c : cfgs) models_.emplace_back(new Model(c));
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
+            "hipStreamCreate");
+  hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
+  hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
+  for (auto &m : models_) points_ += m->grid_points();
+  build_tables();
+}
+
+Batch::~Batch() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  clear_graphs();
+  if (descs_) (void)hipFree(descs_);
+  if (tiles_) (void)hipFree(tiles_);
+  for (auto e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+Model &Batch::model(int m) {
+  if (m < 0 || m >= size()) throw ValueError("model index out of range");
+  return *models_[m];
+}
+
+const Model &Batch::model(int m) const {
+  if (m < 0 || m >= size()) throw ValueError("model index out of range");
+  return *models_[m];
+}
+
+void Batch::clear_graphs() {
+  for (auto g : graphs_)
+    if (g) (void)hipGraphExecDestroy(g);
+  graphs_.clear();
+}
+
+void Batch::set_graph_mode(bool on) {
+  if (!on) clear_graphs();
+  graph_mode_ = on;
+}
+
+void Batch::set_variant(int variant, int xcd_remap) {
+  if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
+      variant != kTwoStepTz16)
+    throw ValueError("batched runs use step variants 0, 1, 6 or 7");
+  variant_ = variant;
+  xcd_remap_ = xcd_remap ? 1 : 0;
+  clear_graphs();
+  build_tables();
+}
+
+void Batch::build_tables() {
+  std::vector<int2> tiles;
+  std::vector<StepArgs> descs;
+  for (size_t m = 0; m < models_.size(); ++m) {
+    models_[m]->set_step_variant(variant_, xcd_remap_);
+    for (int t = 0; t < models_[m]->tile_count(); ++t)
+      tiles.push_back(make_int2((int)m, t));
+    for (int s = 0; s < n_shots_; ++s)
+      for (int p = 0; p < 2; ++p) descs.push_back(models_[m]->forward_launch_args(s, p));
+  }
+  if (tiles.size() > (size_t)std::numeric_limits<int>::max())
+    throw ValueError("too many tiles in one batch");
+  ntiles_ = (int)tiles.size();
+  hip_check(hipStreamSynchronize(stream_), "sync");
+  if (descs_) hip_check(hipFree(descs_), "hipFree");
+  if (tiles_) hip_check(hipFree(tiles_), "hipFree");
+  descs_ = nullptr;
+  tiles_ = nullptr;
+  hip_check(hipMalloc((void **)&descs_, std::max<size_t>(1, descs.size()) * sizeof(StepArgs)),
+            "hipMalloc");
+  hip_check(hipMalloc((void **)&tiles_, std::max<size_t>(1, tiles.size()) * sizeof(int2)),
+            "hipMalloc");
+  hip_check(hipMemcpy(descs_, descs.data(), descs.size() * sizeof(StepArgs),
+                      hipMemcpyHostToDevice),
+            "upload");
+  hip_check(hipMemcpy(tiles_, tiles.data(), tiles.size() * sizeof(int2),
+                      hipMemcpyHostToDevice),
+            "upload");
+}
+
+// Zero every member's state, then advance `nsteps` steps of shot i_shot
+// with one launch per step (or per pair of steps), then form the receiver
+// running sums when the two-step kernels recorded increments.
+void Batch::enqueue_loop(int i_shot, int nsteps) {
+  for (auto &m : models_) m->enqueue_zero_state(stream_);
+  const bool two = step_is_two_step(variant_);
+  int launch = 0;
+  for (int k = 0; k < nsteps;) {
+    const bool pair = two && k + 1 < nsteps;
+    BatchLaunch b{descs_, tiles_, ntiles_, n_shots_, i_shot, launch & 1,
+                  k, k + 1, xcd_remap_};
+    hip_check(launch_batch_step(variant_, pair, b, stream_), "batch step");
+    k += pair ? 2 : 1;
+    ++launch;
+  }
+  if (two && nsteps == nt_)
+    for (auto &m : models_) m->enqueue_receiver_scan(i_shot, stream_);
+}
+
+void Batch::forward(int i_shot) {
+  if (i_shot < 0 || i_shot >= n_shots_)
+    throw ValueError("i_shot is outside [0, n_shots)");
+  hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+  if (graph_mode_) {
+    if (graphs_.size() != (size_t)n_shots_) graphs_.assign(n_shots_, nullptr);
+    if (!graphs_[i_shot]) {
+      hip_check(hipStreamSynchronize(stream_), "sync");
+      hipGraph_t graph = nullptr;
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture");
+      try {
+        enqueue_loop(i_shot, nt_);
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(stream_, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hip_check(e, "hipGraphInstantiate");
+      graphs_[i_shot] = exec;
+      hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+    }
+    hip_check(hipGraphLaunch(graphs_[i_shot], stream_), "hipGraphLaunch");
+  } else {
+    enqueue_loop(i_shot, nt_);
+  }
+  hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+  loop_ms_ += ms;
+}
+
+void Batch::misfits(double *out) {
+  for (size_t m = 0; m < models_.size(); ++m)
+    out[m] = models_[m]->calculate_l2_misfit();
+}
+
+double Batch::autotune(int steps, int *best, double *ms_per_variant) {
+  if (steps < 2) throw ValueError("autotune needs at least 2 steps");
+  const int candidates[4] = {kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16};
+  const int keep_remap = xcd_remap_;
+  double best_ms = std::numeric_limits<double>::infinity();
+  int best_v = variant_;
+  for (int k = 0; k < 4; ++k) {
+    set_variant(candidates[k], keep_remap);
+    enqueue_loop(0, std::min(steps, nt_));  // warm up
+    hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+    enqueue_loop(0, std::min(steps, nt_));
+    hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+    double per = ms / std::min(steps, nt_);
+    if (ms_per_variant) ms_per_variant[k] = per;
+    if (per < best_ms) {
+      best_ms = per;
+      best_v = candidates[k];
+    }
+  }
+  set_variant(best_v, keep_remap);
+  if (best) *best = best_v;
+  return best_ms;
+}
+
+}  // namespace psv

@@ -8,6 +8,7 @@
 #include <new>
 #include <string>
 
+#include "psv_batch.h"
 #include "psv_model.h"
 
 struct psv_model {
@@ -595,6 +596,128 @@ int psv_calibrate_copy(int64_t bytes, int iters, double *ms) {
       throw;
     }
     cleanup();
+  });
+}
+
+struct psv_batch {
+  psv::Batch *impl;
+};
+
+static psv::Batch &B(psv_batch *b) {
+  if (!b || !b->impl) throw psv::ValueError("null batch handle");
+  return *b->impl;
+}
+static const psv::Batch &B(const psv_batch *b) {
+  if (!b || !b->impl) throw psv::ValueError("null batch handle");
+  return *b->impl;
+}
+
+int psv_batch_create(int n, const psv_params *params, psv_batch **out) {
+  return guard([&] {
+    need(out, "out");
+    if (n < 1) throw psv::ValueError("a batch needs at least one model");
+    need(params, "params");
+    std::vector<psv::Config> cfgs;
+    for (int m = 0; m < n; ++m) {
+      psv::Config c = config_from_params(&params[m]);
+      psv::derive(c);  // validate before any device allocation
+      cfgs.push_back(c);
+    }
+    psv_batch *h = new psv_batch;
+    try {
+      h->impl = new psv::Batch(cfgs);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+void psv_batch_destroy(psv_batch *b) {
+  if (!b) return;
+  delete b->impl;
+  delete b;
+}
+
+int psv_batch_size(const psv_batch *b, int *n) {
+  return guard([&] {
+    need(n, "n");
+    *n = B(b).size();
+  });
+}
+
+int psv_batch_get_info(const psv_batch *b, int m, psv_info *out) {
+  return guard([&] {
+    need(out, "out");
+    const psv::Model &mm = B(b).model(m);
+    fill_info(mm.config(), mm.derived(), mm.misfit(), out);
+  });
+}
+
+int psv_batch_set_parameter_fields(psv_batch *b, int m, const double *vp,
+                                   const double *vs, const double *rho) {
+  return guard([&] {
+    need(vp, "vp");
+    need(vs, "vs");
+    need(rho, "rho");
+    B(b).model(m).set_parameter_fields(vp, vs, rho);
+  });
+}
+
+int psv_batch_set_observed_data(psv_batch *b, int m, const double *ux,
+                                const double *uz) {
+  return guard([&] {
+    need(ux, "ux");
+    need(uz, "uz");
+    B(b).model(m).set_traces(1, ux, uz);
+  });
+}
+
+int psv_batch_get_synthetic_data(const psv_batch *b, int m, double *ux,
+                                 double *uz) {
+  return guard([&] {
+    need(ux, "ux");
+    need(uz, "uz");
+    B(b).model(m).get_traces(0, ux, uz);
+  });
+}
+
+int psv_batch_forward(psv_batch *b, int i_shot) {
+  return guard([&] { B(b).forward(i_shot); });
+}
+
+int psv_batch_misfits(psv_batch *b, double *out) {
+  return guard([&] {
+    need(out, "out");
+    B(b).misfits(out);
+  });
+}
+
+int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap) {
+  return guard([&] { B(b).set_variant(variant, xcd_remap); });
+}
+
+int psv_batch_set_graph_mode(psv_batch *b, int on) {
+  return guard([&] { B(b).set_graph_mode(on != 0); });
+}
+
+int psv_batch_autotune(psv_batch *b, int steps, int *best,
+                       double *ms_per_step) {
+  return guard([&] {
+    need(best, "best");
+    B(b).autotune(steps, best, ms_per_step);
+  });
+}
+
+int psv_batch_timing(psv_batch *b, double *loop_ms, int64_t *points,
+                     int reset) {
+  return guard([&] {
+    need(loop_ms, "loop_ms");
+    need(points, "points");
+    *loop_ms = B(b).loop_ms();
+    *points = B(b).grid_points();
+    if (reset) B(b).reset_timing();
   });
 }
 

@@ -195,6 +195,17 @@ int step_single_variant(int variant);
 // Two time steps (a.it, a.it1) in one launch; variant 6 or 7.
 hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t stream,
                             hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// One time step (or two, for the two-step variants) of every tile of a batch
+// of models: descs holds per (model, shot, parity) launch arguments, tiles
+// the (model, tile) pairs of all models.
+struct BatchLaunch {
+  const StepArgs *descs;
+  const int2 *tiles;
+  int ntiles, n_shots, shot, parity, it, it1, xcd_remap;
+};
+hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
+                             hipStream_t stream, hipEvent_t e0 = nullptr,
+                             hipEvent_t e1 = nullptr);
 // Running sums of recorded receiver increments, in place: [nr][nt] x 2.
 hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
                                 hipStream_t s);

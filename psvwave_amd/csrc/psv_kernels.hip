@@ -567,6 +567,34 @@ __global__ void __launch_bounds__(NT) step_two(StepArgs a) {
   two_step_tile<TZ, NT>(a, tile);
 }
 
+__device__ inline int batch_tile(const BatchLaunch &b) {
+  int blk = blockIdx.x;
+  if (!b.xcd_remap) return blk;
+  int chunk = (b.ntiles + 7) / 8;
+  return (blk % 8) * chunk + blk / 8;
+}
+
+template <int TZ, int NT>
+__global__ void __launch_bounds__(NT) batch_fused(BatchLaunch b) {
+  const int g = batch_tile(b);
+  if (g >= b.ntiles) return;
+  const int2 mt = b.tiles[g];
+  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  a.it = b.it;
+  fused_tile<TZ, false, NT>(a, mt.y);
+}
+
+template <int TZ, int NT>
+__global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
+  const int g = batch_tile(b);
+  if (g >= b.ntiles) return;
+  const int2 mt = b.tiles[g];
+  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  a.it = b.it;
+  a.it1 = b.it1;
+  two_step_tile<TZ, NT>(a, mt.y);
+}
+
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
 // thread per trace.
 __global__ void receiver_scan_kernel(double *rx, double *rz, int nr, int nt) {
@@ -881,6 +909,25 @@ hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
     hipExtLaunchKernelGGL((step_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
+                             hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  int nblocks = b.xcd_remap ? 8 * ((b.ntiles + 7) / 8) : b.ntiles;
+  dim3 g(nblocks);
+  const bool tall = step_tile_height(variant) == 16;
+  if (two) {
+    if (tall)
+      hipExtLaunchKernelGGL((batch_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
+    else
+      hipExtLaunchKernelGGL((batch_two<8, 256>), g, dim3(256), 0, s, e0, e1, 0, b);
+  } else {
+    if (tall)
+      hipExtLaunchKernelGGL((batch_fused<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
+    else
+      hipExtLaunchKernelGGL((batch_fused<8, 256>), g, dim3(256), 0, s, e0, e1, 0, b);
+  }
   return hipGetLastError();
 }
 

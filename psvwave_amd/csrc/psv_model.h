@@ -77,6 +77,16 @@ class Model {
   void reset_step_timing() { timed_ms_ = 0, timed_ = 0, launches_ = 0; }
   int64_t device_bytes() const { return device_bytes_; }
 
+  // Pieces of a forward time loop for batched launches across models
+  // (psv_batch.hip): the arguments of a launch that reads state buffer set
+  // `parity`, zeroing the state, and the receiver scan of two-step loops.
+  StepArgs forward_launch_args(int i_shot, int parity) const;
+  void enqueue_zero_state(hipStream_t s) const;
+  void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
+  int tile_count() const { return ntiles_; }
+  int step_variant() const { return variant_; }
+  long long grid_points() const { return ngrid_; }
+
  private:
   void allocate();
   void build_injection_tables();

@@ -591,6 +591,49 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
   }
 }
 
+StepArgs Model::forward_launch_args(int i_shot, int parity) const {
+  check_shot(i_shot);
+  StepArgs a = base_args();
+  a.adjoint = 0;
+  a.dts = cfg_.dt;
+  a.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.entries = fwd_entries_;
+  a.halo_off = fwd_halo_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.halo_entries = fwd_halo_;
+  a.rtf_x = syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.rtf_z = syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.rec_increments = step_is_two_step(variant_) ? 1 : 0;
+  a.rec_off = adj_off_;
+  a.rec_entries = adj_entries_;
+  const int cur = parity & 1, nxt = 1 - cur;
+  a.vx_i = state_[cur][0];
+  a.vz_i = state_[cur][1];
+  a.txx_i = state_[cur][2];
+  a.tzz_i = state_[cur][3];
+  a.txz_i = state_[cur][4];
+  a.vx_o = state_[nxt][0];
+  a.vz_o = state_[nxt][1];
+  a.txx_o = state_[nxt][2];
+  a.tzz_o = state_[nxt][3];
+  a.txz_o = state_[nxt][4];
+  a.do_snap = 0;
+  a.do_snap1 = 0;
+  return a;
+}
+
+void Model::enqueue_zero_state(hipStream_t s) const {
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(state_[k][f], npad_, 0.0, s), "zero state");
+}
+
+void Model::enqueue_receiver_scan(int i_shot, hipStream_t s) const {
+  hip_check(launch_receiver_scan(syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt,
+                                 syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt,
+                                 cfg_.nr, cfg_.nt, s),
+            "receiver scan");
+}
+
 void Model::forward(int i_shot, bool store_fields, bool verbose,
                     bool output_wavefields) {
   check_shot(i_shot);

@@ -83,3 +83,59 @@ def algorithmic_bytes_per_shot(nx, nz, nt, si, kernel_points):
     three kernels)."""
     n = nx * nz
     return n * nt * 208 + (nt // si) * (40 * n + 88 * kernel_points)
+
+
+def gaussian_random_field(shape, corr_len, rng):
+    """Zero-mean, unit-range smooth random field: white noise on a grid
+    coarsened by corr_len / 4, Gaussian-smoothed (sigma = 4 coarse cells),
+    interpolated back, then scaled to [0, 1]."""
+    from scipy import ndimage
+    f = max(1, int(corr_len // 4))
+    coarse = (shape[0] // f + 3, shape[1] // f + 3)
+    g = ndimage.gaussian_filter(rng.standard_normal(coarse), sigma=corr_len / f / 4.0,
+                                mode="wrap")
+    full = ndimage.zoom(g, f, order=1)[: shape[0], : shape[1]]
+    lo, hi = full.min(), full.max()
+    return (full - lo) / (hi - lo) if hi > lo else np.zeros(shape)
+
+
+def monte_carlo_configs(n_models=512, seed=3, nt=3000, np_boundary=30, nr=32):
+    """Configurations of the Monte-Carlo batch: nx_inner ~ U{200..800},
+    nz_inner ~ U{100..400}, dx = dz = 5 m, dt = 0.5 ms, 10 Hz, 2 shots of
+    one source each (at a quarter and three quarters of the width, depth 10),
+    nr receivers evenly along the surface at depth 5."""
+    rng = np.random.default_rng(seed)
+    sizes = [(int(rng.integers(200, 801)), int(rng.integers(100, 401)))
+             for _ in range(n_models)]
+    cfgs = []
+    for nxi, nzi in sizes:
+        cfgs.append({
+            "domain": dict(nt=nt, nx_inner=nxi, nz_inner=nzi, nx_inner_boundary=0,
+                           nz_inner_boundary=0, dx=5.0, dz=5.0, dt=5e-4),
+            "boundary": dict(np_boundary=np_boundary, np_factor=0.015),
+            "medium": dict(scalar_rho=1800.0, scalar_vp=2400.0, scalar_vs=1200.0),
+            "sources": dict(peak_frequency=10.0, n_sources=2, n_shots=2,
+                            source_timeshift=0.0, delay_cycles_per_shot=0,
+                            moment_angles=[45.0, 135.0],
+                            ix_sources=[nxi // 4, (3 * nxi) // 4], iz_sources=[10, 10],
+                            which_source_to_fire_in_which_shot=[[0], [1]]),
+            "receivers": dict(nr=nr,
+                              ix_receivers=[int((k + 0.5) * nxi / nr) for k in range(nr)],
+                              iz_receivers=[5] * nr),
+            "inversion": dict(snapshot_interval=10),
+            "basis": dict(npx=1, npz=1),
+            "output": dict(observed_data_folder=".", stf_folder="."),
+        })
+    return cfgs
+
+
+def monte_carlo_model(cfg, rng, corr_len=20):
+    """(vp, vs, rho) on the full grid: Gaussian random fields with
+    correlation length corr_len points, vp in 1800-3000 m/s, vs/vp in
+    0.45-0.55, rho in 1400-2200 kg/m3."""
+    d, npb = cfg["domain"], cfg["boundary"]["np_boundary"]
+    shape = (d["nx_inner"] + 2 * npb, d["nz_inner"] + 2 * npb)
+    vp = 1800.0 + 1200.0 * gaussian_random_field(shape, corr_len, rng)
+    vs = vp * (0.45 + 0.10 * gaussian_random_field(shape, corr_len, rng))
+    rho = 1400.0 + 800.0 * gaussian_random_field(shape, corr_len, rng)
+    return vp, vs, rho

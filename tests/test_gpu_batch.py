@@ -1,0 +1,85 @@
+"""Batched forward runs: every member must equal the same model run alone
+(bit for bit), for each batched step kernel and in graph mode."""
+import numpy as np
+import pytest
+
+from conftest import small_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _members():
+    sizes = [(90, 50, 12), (37, 13, 0), (130, 21, 7), (200, 64, 10), (64, 16, 4)]
+    cfgs = []
+    for k, (nxi, nzi, npb) in enumerate(sizes):
+        c = small_config(**{"domain.nt": 97, "domain.nx_inner": nxi,
+                            "domain.nz_inner": nzi, "boundary.np_boundary": npb,
+                            "domain.nx_inner_boundary": 1,
+                            "domain.nz_inner_boundary": 1})
+        c["sources"].update(n_sources=2, n_shots=2, moment_angles=[20 + 30 * k, 200],
+                            ix_sources=[max(1, nxi // 3), max(1, nxi // 2)],
+                            iz_sources=[max(1, nzi // 3), max(1, nzi // 2)],
+                            which_source_to_fire_in_which_shot=[[0], [1]])
+        c["receivers"].update(nr=3, ix_receivers=[0, nxi // 2, nxi - 1],
+                              iz_receivers=[nzi - 1, nzi // 2, 0])
+        cfgs.append(c)
+    return cfgs
+
+
+def _fields(nx, nz, seed):
+    rng = np.random.default_rng(seed)
+    return (2000 + 200 * rng.random((nx, nz)), 800 + 100 * rng.random((nx, nz)),
+            1500 + 100 * rng.random((nx, nz)))
+
+
+@pytest.mark.parametrize("variant,graph", [(0, False), (1, False), (6, False),
+                                           (7, False), (6, True)])
+def test_batch_members_equal_single_models(oracle_lib, variant, graph):
+    import psvwave_amd
+    cfgs = _members()
+    batch = psvwave_amd.Batch(cfgs)
+    batch.set_variant(variant)
+    if graph:
+        batch.set_graph_mode(True)
+    singles = []
+    for m, cfg in enumerate(cfgs):
+        info = batch.infos[m]
+        f = _fields(info["nx"], info["nz"], m)
+        batch.set_parameter_fields(m, *f)
+        single = psvwave_amd.fdModel(cfg)
+        single.set_parameter_fields(*f)
+        obs = tuple(np.full((2, 3, 97), 1e-9 * (m + 1)) for _ in range(2))
+        batch.set_observed_data(m, *obs)
+        single.set_observed_data(*obs)
+        singles.append(single)
+    for rep in range(2 if graph else 1):
+        batch.forward_all()
+        misfits = batch.misfits()
+        for m, single in enumerate(singles):
+            for s in range(2):
+                single.forward_simulate(s, store_fields=False)
+            single.calculate_l2_misfit()
+            for a, b in zip(batch.get_synthetic_data(m), single.get_synthetic_data()):
+                assert np.array_equal(a, b), (variant, m, rep)
+            assert misfits[m] == single.misfit
+    # one member against the oracle
+    o = oracle_lib.OracleModel(cfgs[3])
+    o.set_parameter_fields(*_fields(o.nx, o.nz, 3))
+    for s in range(2):
+        o.forward_simulate(s, store_fields=False)
+    for a, b in zip(batch.get_synthetic_data(3), o.get_synthetic_data()):
+        assert np.array_equal(a, b)
+
+
+def test_batch_autotune_and_validation():
+    import psvwave_amd
+    cfgs = _members()
+    batch = psvwave_amd.Batch(cfgs[:2])
+    best, times = batch.autotune(steps=6)
+    assert best in (0, 1, 6, 7) and set(times) == {0, 1, 6, 7}
+    assert all(t > 0 for t in times.values())
+    bad = small_config(**{"domain.nt": 50})
+    with pytest.raises(ValueError, match="share nt"):
+        psvwave_amd.Batch([cfgs[0], bad])
+    with pytest.raises(ValueError, match="variants"):
+        batch.set_variant(2)
