@@ -194,6 +194,14 @@ int psv_set_timing(psv_model *m, int timing_stride);
  * HIP graph and replay it; with timing enabled, each replay is bracketed as
  * a whole (kernel time plus inter-kernel gaps). */
 int psv_set_graph_mode(psv_model *m, int on);
+/* Checkpointing (no reference counterpart; the reference stores every
+ * snapshot, src/fdModel.cpp:555-572): keep only every `every`-th forward
+ * snapshot (1: all, the default) and recompute the others during
+ * adjoint_simulate, one segment of `every` snapshot intervals at a time.
+ * Kernels are bit-identical to full storage; snapshot memory drops by
+ * `every`, the adjoint costs one extra forward.  get_snapshots is then
+ * unavailable.  Changing it discards stored snapshots. */
+int psv_set_checkpointing(psv_model *m, int every);
 int psv_get_step_timing(psv_model *m, double *total_ms, int64_t *timed,
                         int64_t *launches);
 int psv_reset_step_timing(psv_model *m);

@@ -107,6 +107,7 @@ _SIGNATURES = {
     "psv_set_step_variant": [_P, _i, _i],
     "psv_set_timing": [_P, _i],
     "psv_set_graph_mode": [_P, _i],
+    "psv_set_checkpointing": [_P, _i],
     "psv_get_step_timing": [_P, _D, C.POINTER(C.c_int64),
                             C.POINTER(C.c_int64)],
     "psv_reset_step_timing": [_P],

@@ -528,6 +528,10 @@ int psv_set_graph_mode(psv_model *m, int on) {
   return guard([&] { M(m).set_graph_mode(on != 0); });
 }
 
+int psv_set_checkpointing(psv_model *m, int every) {
+  return guard([&] { M(m).set_checkpointing(every); });
+}
+
 int psv_set_timing(psv_model *m, int stride) {
   return guard([&] {
     if (stride < 0) throw psv::ValueError("timing stride must be >= 0");

@@ -245,6 +245,9 @@ hipError_t launch_transpose(const double *in, double *out, int nx, int nz,
 // calibration.
 hipError_t launch_copy(const double *in, double *out, long long n,
                        hipStream_t s);
+// Compact [iz][ix] -> padded stencil array (grid points only).
+hipError_t launch_pad(const double *in, double *out, int nx, int nz,
+                      int pitch, long long origin, hipStream_t s);
 // Padded stencil array -> compact [iz][ix].
 hipError_t launch_unpad(const double *in, double *out, int nx, int nz,
                         int pitch, long long origin, hipStream_t s);

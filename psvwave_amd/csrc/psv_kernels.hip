@@ -878,6 +878,15 @@ __global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ i
     if (base + k * 256 < n2) out[base + k * 256] = v[k];
 }
 
+__global__ void pad_kernel(const double *in, double *out, int nx, int nz,
+                           int pitch, long long origin) {
+  long long n = (long long)nx * nz;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int z = (int)(i / nx), x = (int)(i - (long long)z * nx);
+  out[origin + (long long)z * pitch + x] = in[i];
+}
+
 inline unsigned blocks_for(long long n, int bs) {
   return (unsigned)((n + bs - 1) / bs);
 }
@@ -1067,6 +1076,14 @@ hipError_t launch_copy(const double *in, double *out, long long n,
   unsigned grid = (unsigned)((n2 + 1023) / 1024);
   hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, s,
                      (const double2 *)in, (double2 *)out, n2);
+  return hipGetLastError();
+}
+
+hipError_t launch_pad(const double *in, double *out, int nx, int nz,
+                      int pitch, long long origin, hipStream_t s) {
+  long long n = (long long)nx * nz;
+  hipLaunchKernelGGL(pad_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     in, out, nx, nz, pitch, origin);
   return hipGetLastError();
 }
 

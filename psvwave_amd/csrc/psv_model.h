@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -69,6 +70,10 @@ class Model {
   // Capture each (direction, shot, store) time loop once as a HIP graph and
   // replay it; step timing then brackets whole loops.
   void set_graph_mode(bool on);
+  // Keep every `every`-th forward snapshot only (1: all) and recompute the
+  // others during the adjoint, one segment at a time.
+  void set_checkpointing(int every);
+  int checkpointing() const { return ckpt_every_; }
   void step_timing(double *ms, int64_t *timed, int64_t *launches) const {
     *ms = timed_ms_;
     *timed = timed_;
@@ -139,9 +144,24 @@ class Model {
   bool graph_mode_ = false;
   std::map<long long, hipGraphExec_t> graphs_;
   void clear_graphs();
+  using SlotFn = std::function<bool(int, double **)>;
+  int enqueue_range(StepArgs &a, double *(*st)[5], int parity, bool forward,
+                    int k0, int k1, bool snaps, const SlotFn &slot,
+                    std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
+                    bool dump);
   void enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
                     std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
                     bool dump);
+  void enqueue_checkpointed_adjoint(
+      StepArgs &aa, int i_shot,
+      std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs);
+  bool stored_slot(int i_shot, int it, double **dst) const;
+  int stored_snapshots() const;
+  int ckpt_every_ = 1;
+  double *state2_[2][5] = {};  // forward recomputation buffers (padded)
+  double *seg_[5] = {};        // snapshots of one recomputed segment
+  int seg_k_ = 0;              // slots allocated in seg_
+  int *scratch_off_ = nullptr; // all-zero CSR offsets
 };
 
 }  // namespace psv

@@ -73,9 +73,10 @@ Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
     cp(obs_[k], o.obs_[k], nrec_);
     cp(adj_[k], o.adj_[k], nrec_);
   }
+  ckpt_every_ = o.ckpt_every_;
   if (o.snap_[0]) {
     ensure_snapshots();
-    long long n = (long long)cfg_.n_shots * d_.snapshots * ngrid_;
+    long long n = (long long)cfg_.n_shots * stored_snapshots() * ngrid_;
     for (int f = 0; f < 5; ++f) cp(snap_[f], o.snap_[f], n);
   }
   misfit_ = o.misfit_;
@@ -333,7 +334,7 @@ void Model::set_graph_mode(bool on) {
 void Model::ensure_snapshots() {
   if (snap_[0]) return;
   clear_graphs();
-  long long n = (long long)cfg_.n_shots * d_.snapshots * ngrid_;
+  long long n = (long long)cfg_.n_shots * stored_snapshots() * ngrid_;
   for (int f = 0; f < 5; ++f) snap_[f] = dalloc(n);
 }
 
@@ -424,27 +425,23 @@ StepArgs Model::base_args() const {
   return a;
 }
 
-// Enqueues one time loop: zero the state, then one launch per time step
-// (two for the split variants) or per pair of steps (two-step variants; an
-// odd last step uses the matching one-step kernel).  With `pairs`, every
-// timing_stride-th launch records its kernel start and end; with `dump`, the
-// reference's text snapshot is written every 10 steps (one-step launches).
-void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
+// Enqueues the launches of loop indices [k0, k1) (time step it = k
+// forward, nt - 1 - k adjoint) on state buffer sets st[2][5], starting from
+// set `parity`; returns the set holding the state afterwards.  One launch per
+// step (two for the split variants) or per pair of steps (two-step variants;
+// an unpaired step uses the matching one-step kernel).  `slot(it, ptrs)`
+// decides whether snapshot step it is stored / correlated and where.  With
+// `pairs`, every timing_stride-th launch records its kernel start and end;
+// with `dump`, the reference's text snapshot is written every 10 steps.
+int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
+                         bool forward, int k0, int k1, bool snaps,
+                         const SlotFn &slot,
                          std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
                          bool dump) {
   const bool inplace = step_is_inplace(variant_);
   const bool two = step_is_two_step(variant_) && !dump;
-  const int single = two ? step_single_variant(variant_)
-                         : (dump ? step_single_variant(variant_) : variant_);
+  const int single = (two || dump) ? step_single_variant(variant_) : variant_;
   const int nt = cfg_.nt, si = cfg_.snapshot_interval;
-  // zero the state (src/fdModel.cpp:530-542, 792-805) with a fill kernel,
-  // which is safe inside a captured graph
-  for (int k = 0; k < (inplace ? 1 : 2); ++k)
-    for (int f = 0; f < 5; ++f)
-      hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
-  a.rec_increments = (forward && step_is_two_step(variant_)) ? 1 : 0;
-  a.rec_off = adj_off_;
-  a.rec_entries = adj_entries_;
   size_t next_event = 0;
   auto event = [&]() {
     if (next_event == events_.size()) {
@@ -454,33 +451,26 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
     }
     return events_[next_event++];
   };
-  auto slot = [&](int it, double **dst) {
-    long long off = ((long long)i_shot * d_.snapshots + it / si) * ngrid_;
-    for (int f = 0; f < 5; ++f) dst[f] = snap_[f] + off;
-  };
-  int launch = 0;
-  for (int k = 0; k < nt;) {
-    const int pair = two && k + 1 < nt;
-    const int cur = inplace ? 0 : (launch & 1);
-    const int nxt = inplace ? 0 : ((launch + 1) & 1);
-    a.vx_i = state_[cur][0];
-    a.vz_i = state_[cur][1];
-    a.txx_i = state_[cur][2];
-    a.tzz_i = state_[cur][3];
-    a.txz_i = state_[cur][4];
-    a.vx_o = state_[nxt][0];
-    a.vz_o = state_[nxt][1];
-    a.txx_o = state_[nxt][2];
-    a.tzz_o = state_[nxt][3];
-    a.txz_o = state_[nxt][4];
+  for (int k = k0; k < k1;) {
+    const int pair = two && k + 1 < k1;
+    const int cur = inplace ? 0 : parity;
+    const int nxt = inplace ? 0 : 1 - parity;
+    a.vx_i = st[cur][0];
+    a.vz_i = st[cur][1];
+    a.txx_i = st[cur][2];
+    a.tzz_i = st[cur][3];
+    a.txz_i = st[cur][4];
+    a.vx_o = st[nxt][0];
+    a.vz_o = st[nxt][1];
+    a.txx_o = st[nxt][2];
+    a.tzz_o = st[nxt][3];
+    a.txz_o = st[nxt][4];
     const int it = forward ? k : nt - 1 - k;
     const int it1 = forward ? k + 1 : nt - 2 - k;
     a.it = it;
-    a.do_snap = (it % si == 0) && (store || !forward);
-    if (a.do_snap) slot(it, a.snap);
+    a.do_snap = snaps && (it % si == 0) && slot(it, a.snap);
     a.it1 = it1;
-    a.do_snap1 = pair && (it1 % si == 0) && (store || !forward);
-    if (a.do_snap1) slot(it1, a.snap1);
+    a.do_snap1 = pair && snaps && (it1 % si == 0) && slot(it1, a.snap1);
     const int covered = pair ? 2 : 1;
     bool timed = pairs && timing_stride_ > 0 && (launches_ / covered) % timing_stride_ == 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -495,15 +485,16 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
       hip_check(launch_step(single, a, stream_, e0, e1), "step kernel");
     launches_ += covered;
     if (timed) timed_ += covered;
+    if (!inplace) parity = nxt;
     if (dump && it % 10 == 0) {
       // src/fdModel.cpp:753-776: state after step it, default stream format
       std::vector<double> hx(ngrid_), hz(ngrid_);
       double *s = staging(ngrid_);
-      hip_check(launch_unpad(state_[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
+      hip_check(launch_unpad(st[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
                              stream_),
                 "unpad");
       to_host_layout(hx.data(), s);
-      hip_check(launch_unpad(state_[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
+      hip_check(launch_unpad(st[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
                              stream_),
                 "unpad");
       to_host_layout(hz.data(), s);
@@ -519,11 +510,111 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
       }
     }
     k += covered;
-    ++launch;
   }
+  return parity;
+}
+
+// Snapshot slot of time step it in the stored snapshots: every snapshot,
+// or with checkpointing every ckpt_every_-th one.
+bool Model::stored_slot(int i_shot, int it, double **dst) const {
+  const int slot = it / cfg_.snapshot_interval;
+  if (slot % ckpt_every_ != 0) return false;
+  long long off =
+      ((long long)i_shot * stored_snapshots() + slot / ckpt_every_) * ngrid_;
+  for (int f = 0; f < 5; ++f) dst[f] = snap_[f] + off;
+  return true;
+}
+
+int Model::stored_snapshots() const {
+  return (d_.snapshots + ckpt_every_ - 1) / ckpt_every_;
+}
+
+// One whole time loop from a zero state (src/fdModel.cpp:530-542, 792-805;
+// zeroed with fill kernels, which are safe inside a captured graph).
+void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
+                         std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
+                         bool dump) {
+  const bool inplace = step_is_inplace(variant_);
+  for (int k = 0; k < (inplace ? 1 : 2); ++k)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(state_[k][f], npad_, 0.0, stream_), "zero state");
+  a.rec_increments = (forward && step_is_two_step(variant_)) ? 1 : 0;
+  a.rec_off = adj_off_;
+  a.rec_entries = adj_entries_;
+  SlotFn slot = [this, i_shot](int it, double **dst) {
+    return stored_slot(i_shot, it, dst);
+  };
+  enqueue_range(a, state_, 0, forward, 0, cfg_.nt, store || !forward, slot,
+                pairs, dump);
   if (a.rec_increments)
-    hip_check(launch_receiver_scan(a.rtf_x, a.rtf_z, cfg_.nr, nt, stream_),
+    hip_check(launch_receiver_scan(a.rtf_x, a.rtf_z, cfg_.nr, cfg_.nt, stream_),
               "receiver scan");
+}
+
+// Adjoint with checkpointing: segments of ckpt_every_ snapshot intervals
+// are processed last to first.  For each, the forward state is restored
+// from the segment's checkpoint into a second buffer pair, the segment's
+// snapshots are recomputed into a small buffer, and the adjoint steps of
+// the segment correlate with them.  The recomputed forward runs the same
+// kernels on the same state, so the kernels equal those of full storage
+// bit for bit.
+void Model::enqueue_checkpointed_adjoint(
+    StepArgs &aa, int i_shot,
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs) {
+  const int nt = cfg_.nt, si = cfg_.snapshot_interval, k = ckpt_every_;
+  if (!state2_[0][0]) {
+    for (int p = 0; p < 2; ++p)
+      for (int f = 0; f < 5; ++f) state2_[p][f] = dalloc(npad_);
+    // all-zero CSR offsets for the finest tiling (64 x 8 tiles)
+    const long long max_tiles = (long long)ntx_ * ((d_.nz + 7) / 8);
+    scratch_off_ = (int *)dalloc((size_t)(max_tiles + 2) / 2 + 1);
+  }
+  if (seg_k_ < k) {
+    for (int f = 0; f < 5; ++f) {
+      if (seg_[f]) {
+        auto it = std::find(allocations_.begin(), allocations_.end(), (void *)seg_[f]);
+        if (it != allocations_.end()) allocations_.erase(it);
+        hip_check(hipFree(seg_[f]), "hipFree");
+        device_bytes_ -= (int64_t)((long long)seg_k_ * ngrid_ * sizeof(double));
+      }
+      seg_[f] = dalloc((size_t)k * ngrid_);
+    }
+    seg_k_ = k;
+  }
+  const bool inplace = step_is_inplace(variant_);
+  for (int p = 0; p < (inplace ? 1 : 2); ++p)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(state_[p][f], npad_, 0.0, stream_), "zero state");
+  StepArgs fa = base_args();
+  fa.adjoint = 0;
+  fa.dts = cfg_.dt;
+  fa.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
+  fa.entries = fwd_entries_;
+  fa.halo_off = fwd_halo_off_ + (size_t)i_shot * (ntiles_ + 1);
+  fa.halo_entries = fwd_halo_;
+  fa.nr = 0;  // the recomputation records no receivers
+  fa.rec_off = scratch_off_;  // all-zero CSR: no entries
+  fa.rec_entries = adj_entries_;
+  const int L = k * si;
+  const int nseg = (nt + L - 1) / L;
+  int parity = 0;
+  for (int c = nseg - 1; c >= 0; --c) {
+    const int t0 = c * L, t1 = std::min(t0 + L, nt);
+    SlotFn seg = [this, c, k, si](int it, double **dst) {
+      long long off = (long long)(it / si - c * k) * ngrid_;
+      for (int f = 0; f < 5; ++f) dst[f] = seg_[f] + off;
+      return true;
+    };
+    double *ck[5];
+    stored_slot(i_shot, t0, ck);
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_pad(ck[f], state2_[0][f], d_.nx, d_.nz, pitch_, origin_,
+                           stream_),
+                "restore checkpoint");
+    enqueue_range(fa, state2_, 0, true, t0, t1, true, seg, nullptr, false);
+    parity = enqueue_range(aa, state_, parity, false, nt - t1, nt - t0, true,
+                           seg, pairs, false);
+  }
 }
 
 // The time loop.  Plain mode launches step by step; graph mode replays a
@@ -678,7 +769,19 @@ void Model::adjoint(int i_shot, bool verbose) {
   a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   auto t0 = std::chrono::steady_clock::now();
-  run_steps(a, i_shot, false, false, false);
+  if (ckpt_every_ > 1) {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    enqueue_checkpointed_adjoint(a, i_shot, &pairs);
+    sync();
+    for (auto &p : pairs) {
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, p.first, p.second),
+                "hipEventElapsedTime");
+      timed_ms_ += ms;
+    }
+  } else {
+    run_steps(a, i_shot, false, false, false);
+  }
   if (verbose) {
     double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
                    .count();
@@ -770,8 +873,27 @@ void Model::set_traces(int which, const double *ux, const double *uz) {
   upload(dst[1], uz, nrec_);
 }
 
+void Model::set_checkpointing(int every) {
+  if (every < 1) throw ValueError("checkpoint interval must be >= 1");
+  if (every == ckpt_every_) return;
+  sync();
+  clear_graphs();
+  for (int f = 0; f < 5; ++f) {
+    if (!snap_[f]) continue;
+    auto it = std::find(allocations_.begin(), allocations_.end(), (void *)snap_[f]);
+    if (it != allocations_.end()) allocations_.erase(it);
+    hip_check(hipFree(snap_[f]), "hipFree");
+    device_bytes_ -= (int64_t)((long long)cfg_.n_shots * stored_snapshots() *
+                               ngrid_ * sizeof(double));
+    snap_[f] = nullptr;
+  }
+  ckpt_every_ = every;
+}
+
 void Model::get_snapshot(int i_shot, int i_snap, double *const out[5]) const {
   check_shot(i_shot);
+  if (ckpt_every_ > 1)
+    throw ValueError("snapshots are not stored with checkpointing enabled");
   if (i_snap < 0 || i_snap >= d_.snapshots)
     throw ValueError("snapshot index out of range");
   if (!snap_[0]) {
@@ -783,6 +905,8 @@ void Model::get_snapshot(int i_shot, int i_snap, double *const out[5]) const {
 }
 
 void Model::get_snapshots(double *const out[5]) const {
+  if (ckpt_every_ > 1)
+    throw ValueError("snapshots are not stored with checkpointing enabled");
   long long slices = (long long)cfg_.n_shots * d_.snapshots;
   if (!snap_[0]) {
     for (int f = 0; f < 5; ++f)

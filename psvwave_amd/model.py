@@ -380,6 +380,12 @@ class fdModel:
         """Replay each shot's time loop as a captured HIP graph."""
         _lib.check(self._lib.psv_set_graph_mode(self._h, int(bool(on))))
 
+    def set_checkpointing(self, every):
+        """Keep every `every`-th forward snapshot only and recompute the
+        rest during adjoint_simulate (bit-identical kernels, 1/every of the
+        snapshot memory, one extra forward per adjoint)."""
+        _lib.check(self._lib.psv_set_checkpointing(self._h, int(every)))
+
     def set_timing(self, stride):
         """Bracket every `stride`-th step launch with HIP events (0: off)."""
         _lib.check(self._lib.psv_set_timing(self._h, int(stride)))
