@@ -247,6 +247,7 @@ def main():
                    3: "step_fused<8,recip>", 4: "step_fused<16,recip>",
                    5: "step_split_stress<8,recip>+step_split_velocity<8,recip>",
                    6: "step_two<8,256>", 7: "step_two<16,512>",
+                   8: "step_lean<16>", 9: "step_lean<8>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -13,6 +13,9 @@ import numpy as np
 from . import _lib
 from .model import _f64
 
+# step variants Batch.autotune times, in the library's order
+AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9)
+
 
 class Batch:
     def __init__(self, configs):
@@ -90,13 +93,14 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 
     def autotune(self, steps=20):
-        """Time variants 0, 1, 6, 7 over `steps` steps and keep the fastest.
+        """Time variants 0, 1, 6, 7, 8, 9 over `steps` steps and keep the fastest.
         Returns (best variant, {variant: ms per step})."""
         best = C.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(len(AUTOTUNE_VARIANTS))
         _lib.check(self._lib.psv_batch_autotune(self._h, int(steps), C.byref(best),
                                                 _lib.dptr(ms)))
-        return best.value, dict(zip((0, 1, 6, 7), ms.tolist()))
+        return best.value, {v: t for v, t in zip(AUTOTUNE_VARIANTS, ms.tolist())
+                            if t >= 0}
 
     def timing(self, reset=False):
         """(summed forward-loop milliseconds, grid points of all members)."""

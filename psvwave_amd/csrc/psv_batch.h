@@ -10,6 +10,9 @@
 
 namespace psv {
 
+// Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9.
+constexpr int kBatchCandidates = 6;
+
 class Batch {
  public:
   // All members must share nt and n_shots.
@@ -23,7 +26,7 @@ class Batch {
   const Model &model(int m) const;
 
   // Step kernel for every member: 0 / 1 one-step 64x8 / 64x16 tiles,
-  // 6 / 7 two-step 64x8 / 64x16 tiles.
+  // 6 / 7 two-step 64x8 / 64x16 tiles, 8 / 9 lean one-step 64x16 / 64x8.
   void set_variant(int variant, int xcd_remap);
   int variant() const { return variant_; }
   void set_graph_mode(bool on);
@@ -31,8 +34,9 @@ class Batch {
   void forward(int i_shot);
   // Per-member L2 misfit over all shots (calculate_l2_misfit of each).
   void misfits(double *out);
-  // Times `steps` batched steps of each candidate variant, keeps the
-  // fastest; returns its time per step in ms.
+  // Times `steps` batched steps of each candidate variant (ms_per_variant
+  // gets kBatchCandidates entries, -1 for a variant the batch cannot use),
+  // keeps the fastest; returns its time per step in ms.
   double autotune(int steps, int *best, double *ms_per_variant);
   long long grid_points() const { return points_; }
   double loop_ms() const { return loop_ms_; }

@@ -34,6 +34,9 @@ Batch::Batch(const std::vector<Config> &cfgs) {
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
   for (auto &m : models_) points_ += m->grid_points();
+  bool lean = true;
+  for (auto &m : models_) lean = lean && m->lean_ok();
+  if (lean) variant_ = kLeanTz16;
   build_tables();
 }
 
@@ -70,8 +73,12 @@ void Batch::set_graph_mode(bool on) {
 
 void Batch::set_variant(int variant, int xcd_remap) {
   if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
-      variant != kTwoStepTz16)
-    throw ValueError("batched runs use step variants 0, 1, 6 or 7");
+      variant != kTwoStepTz16 && !step_is_lean(variant))
+    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8 or 9");
+  if (step_is_lean(variant))
+    for (auto &m : models_)
+      if (!m->lean_ok())
+        throw ValueError("a member of the batch cannot use step variants 8, 9");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
   clear_graphs();
@@ -172,11 +179,16 @@ void Batch::misfits(double *out) {
 
 double Batch::autotune(int steps, int *best, double *ms_per_variant) {
   if (steps < 2) throw ValueError("autotune needs at least 2 steps");
-  const int candidates[4] = {kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16};
+  const int candidates[kBatchCandidates] = {kFusedTz8, kFusedTz16, kTwoStepTz8,
+                                            kTwoStepTz16, kLeanTz16, kLeanTz8};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;
-  for (int k = 0; k < 4; ++k) {
+  bool lean = true;
+  for (auto &m : models_) lean = lean && m->lean_ok();
+  for (int k = 0; k < kBatchCandidates; ++k) {
+    if (ms_per_variant) ms_per_variant[k] = -1.0;
+    if (step_is_lean(candidates[k]) && !lean) continue;
     set_variant(candidates[k], keep_remap);
     enqueue_loop(0, std::min(steps, nt_));  // warm up
     hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");

@@ -16,7 +16,8 @@
 namespace psv {
 
 constexpr int kTileX = 64;   // tile width = one wavefront of lanes
-constexpr int kHaloX = 8;    // margin columns left of x = 0
+constexpr int kHaloX = 16;   // margin columns left of x = 0 (rows stay
+                             // 128-byte aligned at tile starts)
 constexpr int kHaloZ = 8;    // margin rows above z = 0
 constexpr int kBlock = 256;  // threads per block for the step kernels
 
@@ -49,6 +50,11 @@ struct StepArgs {
   double dts;  // +dt forward, -dt adjoint (stencil sign)
   double dt, dx, dz, dxdz, d4x, d4z;
   double rdx, rdz;  // RN(1/dx), RN(1/dz) for RecipDiv
+  // FastDiv split reciprocals: zh = RD(1/d), zl = RN(1/d - zh) >= 0
+  double zhx, zlx, zhz, zlz;
+  // taper factors by column and by row: taper(x, z) = min(taper_x[x],
+  // taper_z[z]) (the table is nondecreasing in the nesting count)
+  const double *taper_x, *taper_z;
   int it, nt, adjoint;
   // snapshot slot of this step (compact [iz][ix]); written forward, read
   // adjoint when do_snap is set
@@ -171,7 +177,8 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 
 // Step-kernel variants: 0-2 divide by the spacings with IEEE divisions,
 // 3-5 are the same kernels with RecipDiv (bit-identical results), 6-7
-// advance two time steps per launch.
+// advance two time steps per launch, 8-9 are the lean one-step kernels
+// (64 x 16 and 64 x 8 tiles, FastDiv with IEEE fallback).
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -181,8 +188,17 @@ enum StepVariant {
   kSplitTz8Recip = 5,
   kTwoStepTz8 = 6,
   kTwoStepTz16 = 7,
-  kNumVariants = 8
+  kLeanTz16 = 8,
+  kLeanTz8 = 9,
+  kNumVariants = 10
 };
+
+// Lean one-step kernels (variants 8, 9): lane = column, FastDiv, reduced
+// stress ring.  They need FastDiv's divisor range and a taper table that is
+// nondecreasing in the nesting count; the host checks both.
+bool step_is_lean(int variant);
+bool fast_division_ok(double d);
+void fast_division_constants(double d, double *zh, double *zl);
 
 // Host launchers (psv_kernels.hip).  All enqueue on `stream`.
 hipError_t launch_step(int variant, const StepArgs &a, hipStream_t stream,

@@ -22,6 +22,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <cmath>
+
 namespace psv {
 
 namespace {
@@ -306,6 +308,401 @@ __global__ void __launch_bounds__(NT) step_fused(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   fused_tile<TZ, RECIP, NT>(a, tile);
+}
+
+// FastDiv: correctly rounded x / d for a constant d > 0 in four FMA-class
+// operations and no reciprocal instruction.  With zh = RD(1/d),
+// zl = RN(1/d - zh) and y = RN(1/d):
+//   q0 = RN(x zl + RN(x zh))   is within half an ulp plus ~2^-104 |x/d| of
+//                              x / d, hence a faithful quotient;
+//   r  = RN(q0 d - x)          is exact (residual of a faithful quotient);
+//   q1 = RN(q0 - r y)          is RN(x / d) (Markstein's correction theorem:
+//                              y within half an ulp of 1/d, q0 faithful).
+// zl >= 0 makes q0 keep the sign of a zero x, and the residual's sign
+// convention keeps it through q1.  The theorem needs no underflow or
+// overflow: that holds for d in [2^-40, 2^40] and x = 0 or
+// 2^-960 <= |x| <= 2^980.  Every call folds x into two flags (smallest
+// binary exponent, sum of magnitudes -- inf and NaN poison the sum); a
+// caller whose flags fail recomputes with IeeeDiv.
+struct DivFlags {
+  int emin;
+  double asum;
+  __device__ DivFlags() : emin(0), asum(0.0) {}
+  __device__ bool ok() const { return emin >= -959 && asum <= 0x1p980; }
+};
+struct FastDiv {
+  double d, y, zh, zl;
+  DivFlags *f;
+  __device__ double operator()(double x) const {
+    f->emin = min(f->emin, __builtin_amdgcn_frexp_exp(x));
+    f->asum += __builtin_fabs(x);
+    double q0 = __builtin_fma(x, zl, x * zh);
+    double r = __builtin_fma(q0, d, -x);
+    return __builtin_fma(-r, y, q0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Lean one-step kernel.  A block of four waves owns a 64 x TZ tile; lane l of
+// wave w owns column x0 + l on rows w*R .. w*R + R - 1 (R = TZ / 4), so every
+// row a wave touches is one coalesced 512-byte segment, row offsets are
+// wave-uniform (scalar) and every LDS access is an immediate offset from one
+// per-lane address.  Global loads address a per-field scalar base plus one
+// 32-bit per-lane row offset shared by all fields.
+//
+// The velocity update at (x, z) reads txx at x-2..x+1 and txz at x-1..x+2 on
+// its row, txz at z-2..z+1 and tzz at z-1..z+2 on its column -- no corner
+// values.  Besides its own points the block therefore updates only
+//   rows    z0-2: txz   z0-1: txz, tzz   z0+TZ: tzz, txz   z0+TZ+1: tzz
+//   columns x0-2: txx   x0-1: txx, txz   x0+64: txx, txz   x0+65: txz
+// (one ring row per wave; the 4*TZ ring-column items in ceil(TZ/16) passes).
+// Velocities are staged in LDS on rows z0-3..z0+TZ+2, columns x0-3..x0+66.
+//
+// Divisions use FastDiv; a lane whose flags fail redoes its pass with IEEE
+// divisions, so results are bitwise those of the IEEE kernels.
+enum : int { kXX = 1, kZZ = 2, kXZ = 4 };
+
+template <int TZ>
+struct Lean {
+  static constexpr int R = TZ / 4;          // own rows per wave
+  static constexpr int VR = TZ + 6, VW = 70;  // v: rows -3.., cols -3..
+  static constexpr int XXW = 67;            // txx: rows 0..TZ-1, cols -2..64
+  static constexpr int ZZR = TZ + 3;        // tzz: rows -1..TZ+1, cols 0..63
+  static constexpr int XZR = TZ + 3, XZW = 67;  // txz: rows -2..TZ, cols -1..65
+  static constexpr int NCI = 4 * TZ;        // ring-column items
+  static constexpr int NCP = (NCI + 63) / 64;  // ring-column passes
+  static constexpr int KV = (VR + 3) / 4;   // v rows per wave (main columns)
+  static constexpr int NSIDE = 6 * VR;      // v side items (3 + 3 columns)
+  __device__ static int V(int r, int c) { return (r + 3) * VW + (c + 3); }
+  __device__ static int XX(int r, int c) { return r * XXW + (c + 2); }
+  __device__ static int ZZ(int r, int c) { return (r + 1) * 64 + c; }
+  __device__ static int XZ(int r, int c) { return (r + 2) * XZW + (c + 1); }
+};
+
+// Stress components MASK at the point whose v-stage index is v, expression
+// trees of stress_point.
+template <int MASK, int VW, class DX, class DZ>
+__device__ __forceinline__ void lean_stress(const double *sx, const double *sz,
+                                            int v, double lm, double mu,
+                                            double taper, double dts, DX divx,
+                                            DZ divz, double &txx, double &tzz,
+                                            double &txz) {
+  if (MASK & (kXX | kZZ)) {
+    double dvx_x = kC1 * (sx[v + 1] - sx[v]) + kC2 * (sx[v - 1] - sx[v + 2]);
+    double dvz_z = kC1 * (sz[v] - sz[v - VW]) + kC2 * (sz[v - 2 * VW] - sz[v + VW]);
+    double la = __builtin_fma(-2.0, mu, lm);  // == lm - 2 * mu (2 mu is exact)
+    if (MASK & kXX) txx = taper * (txx + dts * (divx(lm * dvx_x) + divz(la * dvz_z)));
+    if (MASK & kZZ) tzz = taper * (tzz + dts * (divx(la * dvx_x) + divz(lm * dvz_z)));
+  }
+  if (MASK & kXZ) {
+    double dvx_z = kC1 * (sx[v + VW] - sx[v]) + kC2 * (sx[v - VW] - sx[v + 2 * VW]);
+    double dvz_x = kC1 * (sz[v] - sz[v - 1]) + kC2 * (sz[v - 2] - sz[v + 1]);
+    txz = taper * (txz + dts * mu * (divz(dvx_z) + divx(dvz_x)));
+  }
+}
+
+// Velocity at own point (r, c) from the staged stresses (velocity_point).
+template <int TZ, class DX, class DZ>
+__device__ __forceinline__ void lean_velocity(const double *sxx,
+                                              const double *szz,
+                                              const double *sxz, int r, int c,
+                                              double b, double taper,
+                                              double dts, DX divx, DZ divz,
+                                              double &vx, double &vz) {
+  using L = Lean<TZ>;
+  const int ix = L::XX(r, c), iz = L::ZZ(r, c), ixz = L::XZ(r, c);
+  constexpr int W = L::XZW;
+  double ax = divx(kC1 * (sxx[ix] - sxx[ix - 1]) + kC2 * (sxx[ix - 2] - sxx[ix + 1])) +
+              divz(kC1 * (sxz[ixz] - sxz[ixz - W]) + kC2 * (sxz[ixz - 2 * W] - sxz[ixz + W]));
+  double az = divx(kC1 * (sxz[ixz + 1] - sxz[ixz]) + kC2 * (sxz[ixz - 1] - sxz[ixz + 2])) +
+              divz(kC1 * (szz[iz + 64] - szz[iz]) + kC2 * (szz[iz - 64] - szz[iz + 128]));
+  vx = taper * (vx + b * dts * ax);
+  vz = taper * (vz + b * dts * az);
+}
+
+__device__ __forceinline__ double ldb(const double *base, unsigned off) {
+  return *(const double *)((const char *)base + off);
+}
+__device__ __forceinline__ void stb(double *base, unsigned off, double v) {
+  *(double *)((char *)base + off) = v;
+}
+
+template <int TZ>
+__device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
+  using L = Lean<TZ>;
+  constexpr int R = L::R;
+  __shared__ double s_vx[L::VR * L::VW], s_vz[L::VR * L::VW];
+  __shared__ double s_xx[TZ * L::XXW], s_zz[L::ZZR * 64], s_xz[L::XZR * L::XZW];
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  // Field bases at tile-relative (row -3, column -8): every offset below is
+  // a nonnegative 32-bit byte count.
+  const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 8);
+  const unsigned rowb = (unsigned)a.pitch * 8u;
+  auto off = [&](int r, int c) -> unsigned {
+    return (unsigned)(r + 3) * rowb + (unsigned)(c + 8) * 8u;
+  };
+  const double *vxi = a.vx_i + tb, *vzi = a.vz_i + tb;
+  const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
+  const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
+
+  // ---- phase A: every global load before the first wait ----
+  double vxm[L::KV], vzm[L::KV];
+#pragma unroll
+  for (int k = 0; k < L::KV; ++k) {
+    const int r = -3 + w + 4 * k;
+    vxm[k] = vzm[k] = 0.0;
+    if (r <= TZ + 2) {
+      unsigned o = off(r, lane);
+      vxm[k] = ldb(vxi, o);
+      vzm[k] = ldb(vzi, o);
+    }
+  }
+  double vxs[2], vzs[2];
+  int sidx[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    sidx[k] = -1;
+    vxs[k] = vzs[k] = 0.0;
+    if (t < L::NSIDE) {
+      int r = t / 6 - 3, cs = t - (t / 6) * 6;
+      int c = cs < 3 ? cs - 3 : 61 + cs;
+      unsigned o = off(r, c);
+      vxs[k] = ldb(vxi, o);
+      vzs[k] = ldb(vzi, o);
+      sidx[k] = L::V(r, c);
+    }
+  }
+  // own stress points
+  double oxx[R], ozz[R], oxz[R], olm[R], omu[R], ob[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    unsigned o = off(w * R + j, lane);
+    oxx[j] = ldb(txxi, o);
+    ozz[j] = ldb(tzzi, o);
+    oxz[j] = ldb(txzi, o);
+    olm[j] = ldb(lmp, o);
+    omu[j] = ldb(mup, o);
+    ob[j] = ldb(bp, o);
+  }
+  // this wave's ring row
+  const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
+  double gxx, gzz, gxz, glm, gmu;
+  {
+    unsigned o = off(rr, lane);
+    gxx = ldb(txxi, o);
+    gzz = ldb(tzzi, o);
+    gxz = ldb(txzi, o);
+    glm = ldb(lmp, o);
+    gmu = ldb(mup, o);
+  }
+  // ring-column items handled by this wave: item i = p * 64 + lane
+  double cxx[L::NCP], cxz[L::NCP], clm[L::NCP], cmu[L::NCP];
+#pragma unroll
+  for (int p = 0; p < L::NCP; ++p) {
+    cxx[p] = cxz[p] = clm[p] = cmu[p] = 0.0;
+    const int i = p * 64 + lane;
+    if ((p & 3) == w && i < L::NCI) {
+      const int q = i / TZ, r = i - q * TZ;
+      const int c = q < 2 ? q - 2 : 62 + q;
+      unsigned o = off(r, c);
+      cxx[p] = ldb(txxi, o);
+      cxz[p] = ldb(txzi, o);
+      clm[p] = ldb(lmp, o);
+      cmu[p] = ldb(mup, o);
+    }
+  }
+  // taper factors and interior masks
+  const int x = x0 + lane;
+  const bool xin = x < a.nx;
+  const bool xint = x >= 2 && x < a.nx - 2;
+  const double tx = a.taper_x[xin ? x : 0];
+  if (!a.adjoint && tile == 0) record_receivers(a);
+
+#pragma unroll
+  for (int k = 0; k < L::KV; ++k) {
+    const int r = -3 + w + 4 * k;
+    if (r <= TZ + 2) {
+      s_vx[L::V(r, lane)] = vxm[k];
+      s_vz[L::V(r, lane)] = vzm[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (sidx[k] >= 0) {
+      s_vx[sidx[k]] = vxs[k];
+      s_vz[sidx[k]] = vzs[k];
+    }
+  __syncthreads();
+
+  // ---- phase B: stresses ----
+  const double dts = a.dts;
+  DivFlags fl;
+  const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+  const IeeeDiv ix{a.dx}, iz{a.dz};
+  auto zint = [&](int z) { return z >= 2 && z < a.nz - 2; };
+  auto taper_row = [&](int z) {
+    return a.taper_z[z < 0 ? 0 : (z >= a.nz ? a.nz - 1 : z)];
+  };
+  double nxx[R], nzz[R], nxz[R], trow[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r = w * R + j, z = z0 + r;
+    nxx[j] = oxx[j], nzz[j] = ozz[j], nxz[j] = oxz[j];
+    trow[j] = taper_row(z);
+    if (xint && zint(z))
+      lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), olm[j], omu[j],
+                                          fmin(tx, trow[j]), dts, fx, fz, nxx[j],
+                                          nzz[j], nxz[j]);
+  }
+  if (!fl.ok()) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int r = w * R + j, z = z0 + r;
+      nxx[j] = oxx[j], nzz[j] = ozz[j], nxz[j] = oxz[j];
+      if (xint && zint(z))
+        lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), olm[j], omu[j],
+                                            fmin(tx, trow[j]), dts, ix, iz, nxx[j],
+                                            nzz[j], nxz[j]);
+    }
+  }
+  // ring row
+  {
+    const int z = z0 + rr;
+    const bool in = xint && zint(z);
+    const double t = fmin(tx, taper_row(z));
+    const int v = L::V(rr, lane);
+    double hxx = gxx, hzz = gzz, hxz = gxz;
+    auto ring = [&](auto dx, auto dz) {
+      hxx = gxx, hzz = gzz, hxz = gxz;
+      if (!in) return;
+      if (w == 0)
+        lean_stress<kXZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
+      else if (w == 3)
+        lean_stress<kZZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
+      else
+        lean_stress<kZZ | kXZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
+    };
+    DivFlags gf;
+    const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
+    ring(gx, gz);
+    if (!gf.ok()) ring(ix, iz);
+    if (w != 3) s_xz[L::XZ(rr, lane)] = hxz;
+    if (w != 0) s_zz[L::ZZ(rr, lane)] = hzz;
+  }
+  // ring columns
+#pragma unroll
+  for (int p = 0; p < L::NCP; ++p) {
+    const int i = p * 64 + lane;
+    if ((p & 3) == w && i < L::NCI) {
+      const int q = i / TZ, r = i - q * TZ;
+      const int c = q < 2 ? q - 2 : 62 + q;
+      const int xc = x0 + c, z = z0 + r;
+      const bool in = xc >= 2 && xc < a.nx - 2 && zint(z);
+      double hxx = cxx[p], hzz = 0.0, hxz = cxz[p];
+      if (in) {
+        const double t = fmin(a.taper_x[xc], taper_row(z));
+        DivFlags cf;
+        const FastDiv cx{a.dx, a.rdx, a.zhx, a.zlx, &cf}, cz{a.dz, a.rdz, a.zhz, a.zlz, &cf};
+        lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), clm[p], cmu[p], t, dts, cx,
+                                      cz, hxx, hzz, hxz);
+        if (!cf.ok()) {
+          hxx = cxx[p], hxz = cxz[p];
+          lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), clm[p], cmu[p], t, dts,
+                                        ix, iz, hxx, hzz, hxz);
+        }
+      }
+      if (q != 3) s_xx[L::XX(r, c)] = hxx;
+      if (q != 0) s_xz[L::XZ(r, c)] = hxz;
+    }
+  }
+  // own stresses: LDS, global stores, start-of-step snapshot work
+  double *txxo = a.txx_o + tb, *tzzo = a.tzz_o + tb, *txzo = a.txz_o + tb;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r = w * R + j, z = z0 + r;
+    s_xx[L::XX(r, lane)] = nxx[j];
+    s_zz[L::ZZ(r, lane)] = nzz[j];
+    s_xz[L::XZ(r, lane)] = nxz[j];
+    if (xin && z < a.nz) {
+      unsigned o = off(r, lane);
+      stb(txxo, o, nxx[j]);
+      stb(tzzo, o, nzz[j]);
+      stb(txzo, o, nxz[j]);
+      if (a.do_snap)
+        owner_snapshot(a, a.snap, x, z, s_vx[L::V(r, lane)], s_vz[L::V(r, lane)],
+                       oxx[j], ozz[j], oxz[j]);
+    }
+  }
+  __syncthreads();
+
+  // ---- phase C: velocities ----
+  double vx[R], vz[R];
+  DivFlags vf;
+  const FastDiv ux{a.dx, a.rdx, a.zhx, a.zlx, &vf}, uz{a.dz, a.rdz, a.zhz, a.zlz, &vf};
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r = w * R + j, z = z0 + r;
+    vx[j] = s_vx[L::V(r, lane)];
+    vz[j] = s_vz[L::V(r, lane)];
+    if (xint && zint(z))
+      lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ux,
+                        uz, vx[j], vz[j]);
+  }
+  if (!vf.ok()) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int r = w * R + j, z = z0 + r;
+      vx[j] = s_vx[L::V(r, lane)];
+      vz[j] = s_vz[L::V(r, lane)];
+      if (xint && zint(z))
+        lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ix,
+                          iz, vx[j], vz[j]);
+    }
+  }
+  // injection terms of this tile, in list order, applied by their owners
+  const int e0 = a.tile_off[tile], e1 = a.tile_off[tile + 1];
+  for (int e = e0; e < e1; ++e) {
+    const InjEntry E = a.entries[e];
+    const int r = E.local / kTileX, c = E.local - r * kTileX;
+    if (r / R != w || c != lane) continue;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j == r - w * R) {
+        double term;
+        if (!a.adjoint) {
+          double S = a.stf[(long long)E.src * a.nt + a.it];
+          term = E.coef * S * a.dt * ob[j] / (E.comp == 0 ? a.d4x : a.d4z);
+        } else {
+          const double *src = E.comp == 0 ? a.adj_x : a.adj_z;
+          term = a.dt * ob[j] * src[(long long)E.src * a.nt + a.it] / a.dxdz;
+        }
+        if (E.comp == 0)
+          vx[j] += term;
+        else
+          vz[j] += term;
+      }
+  }
+  double *vxo = a.vx_o + tb, *vzo = a.vz_o + tb;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int z = z0 + w * R + j;
+    if (xin && z < a.nz) {
+      unsigned o = off(w * R + j, lane);
+      stb(vxo, o, vx[j]);
+      stb(vzo, o, vz[j]);
+    }
+  }
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(256) step_lean(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  lean_tile<TZ>(a, tile);
 }
 
 // Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
@@ -593,6 +990,16 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
   a.it = b.it;
   a.it1 = b.it1;
   two_step_tile<TZ, NT>(a, mt.y);
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(256) batch_lean(BatchLaunch b) {
+  const int g = batch_tile(b);
+  if (g >= b.ntiles) return;
+  const int2 mt = b.tiles[g];
+  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  a.it = b.it;
+  lean_tile<TZ>(a, mt.y);
 }
 
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
@@ -895,9 +1302,22 @@ inline unsigned blocks_for(long long n, int bs) {
 
 int step_tile_height(int variant) {
   return (variant == kFusedTz16 || variant == kFusedTz16Recip ||
-          variant == kTwoStepTz16)
+          variant == kTwoStepTz16 || variant == kLeanTz16)
              ? 16
              : 8;
+}
+bool step_is_lean(int variant) {
+  return variant == kLeanTz16 || variant == kLeanTz8;
+}
+bool fast_division_ok(double d) {
+  return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
+}
+void fast_division_constants(double d, double *zh, double *zl) {
+  double y = 1.0 / d;
+  // 1 - y d is exact (y is within an ulp of 1/d); negative means y > 1/d
+  if (std::fma(-y, d, 1.0) < 0.0) y = std::nextafter(y, 0.0);
+  *zh = y;
+  *zl = std::fma(-y, d, 1.0) / d;  // RN of the exact (1 - zh d) / d, >= 0
 }
 bool step_is_two_step(int variant) {
   return variant == kTwoStepTz8 || variant == kTwoStepTz16;
@@ -926,7 +1346,12 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
   int nblocks = b.xcd_remap ? 8 * ((b.ntiles + 7) / 8) : b.ntiles;
   dim3 g(nblocks);
   const bool tall = step_tile_height(variant) == 16;
-  if (two) {
+  if (step_is_lean(variant)) {
+    if (tall)
+      hipExtLaunchKernelGGL((batch_lean<16>), g, dim3(256), 0, s, e0, e1, 0, b);
+    else
+      hipExtLaunchKernelGGL((batch_lean<8>), g, dim3(256), 0, s, e0, e1, 0, b);
+  } else if (two) {
     if (tall)
       hipExtLaunchKernelGGL((batch_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
     else
@@ -978,6 +1403,12 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
     case kSplitTz8Recip:
       hipExtLaunchKernelGGL((step_split_stress<8, true>), g, b, 0, s, e0, nullptr, 0, a);
       hipExtLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, nullptr, e1, 0, a);
+      break;
+    case kLeanTz16:
+      hipExtLaunchKernelGGL((step_lean<16>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz8:
+      hipExtLaunchKernelGGL((step_lean<8>), g, b, 0, s, e0, e1, 0, a);
       break;
     default:
       return hipErrorInvalidValue;

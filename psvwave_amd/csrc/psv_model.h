@@ -89,6 +89,7 @@ class Model {
   void enqueue_zero_state(hipStream_t s) const;
   void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
   int tile_count() const { return ntiles_; }
+  bool lean_ok() const { return lean_ok_; }
   int step_variant() const { return variant_; }
   long long grid_points() const { return ngrid_; }
 
@@ -125,6 +126,10 @@ class Model {
   double *lam_k_ = nullptr, *mu_k_ = nullptr, *rho_l_ = nullptr;
   double *vp_k_ = nullptr, *vs_k_ = nullptr, *rho_v_ = nullptr;
   double *stf_ = nullptr, *taper_lut_ = nullptr;
+  double *taper_x_ = nullptr, *taper_z_ = nullptr;  // per column / per row
+  bool lean_ok_ = false;
+  bool lean_eligible() const;
+  double taper_of_count(int c) const;
   double *syn_[2] = {}, *obs_[2] = {}, *adj_[2] = {};
   double *snap_[5] = {};
   long long *rec_idx_ = nullptr;

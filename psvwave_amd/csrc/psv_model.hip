@@ -36,6 +36,8 @@ std::string zero_pad(int num, int pad) {
 }  // namespace
 
 Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
+  lean_ok_ = lean_eligible();
+  if (lean_ok_) variant_ = kLeanTz16;
   allocate();
   // homogeneous background then Lame fields (src/fdModel.cpp:248-261)
   hip_check(launch_fill(vp_, ngrid_, cfg_.scalar_vp, stream_), "fill vp");
@@ -48,6 +50,7 @@ Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
 }
 
 Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
+  lean_ok_ = o.lean_ok_;
   variant_ = o.variant_;
   xcd_remap_ = o.xcd_remap_;
   allocate();
@@ -142,6 +145,19 @@ void Model::allocate() {
   upload(stf_, d_.stf.data(), d_.stf.size());
   taper_lut_ = dalloc(d_.taper_lut.size());
   upload(taper_lut_, d_.taper_lut.data(), d_.taper_lut.size());
+  {
+    // nesting count along one axis: min(np, 1 + distance to the nearer edge)
+    auto count1 = [&](int i, int n) {
+      return std::min(cfg_.np_boundary, 1 + std::min(i, n - 1 - i));
+    };
+    std::vector<double> tx(d_.nx), tz(d_.nz);
+    for (int x = 0; x < d_.nx; ++x) tx[x] = taper_of_count(count1(x, d_.nx));
+    for (int z = 0; z < d_.nz; ++z) tz[z] = taper_of_count(count1(z, d_.nz));
+    taper_x_ = dalloc(tx.size());
+    upload(taper_x_, tx.data(), tx.size());
+    taper_z_ = dalloc(tz.size());
+    upload(taper_z_, tz.data(), tz.size());
+  }
   std::vector<long long> ridx(std::max(cfg_.nr, 1), 0);
   for (int r = 0; r < cfg_.nr; ++r)
     ridx[r] = origin_ + (long long)d_.iz_rec[r] * pitch_ + d_.ix_rec[r];
@@ -312,9 +328,27 @@ void Model::build_injection_tables() {
   put((void *&)adj_halo_, adj_halo.data(), adj_halo.size() * sizeof(InjEntry));
 }
 
+// Taper of a point with nesting count c (taper_at in the kernels).
+double Model::taper_of_count(int c) const {
+  return c >= cfg_.np_boundary ? 1.0 : d_.taper_lut[c];
+}
+
+// The lean kernels divide with FastDiv and form the taper as the minimum of
+// a column and a row factor; both are exact only for spacings in
+// [2^-40, 2^40] and a taper nondecreasing in the nesting count.
+bool Model::lean_eligible() const {
+  if (!fast_division_ok(cfg_.dx) || !fast_division_ok(cfg_.dz)) return false;
+  for (int c = 0; c < cfg_.np_boundary; ++c)
+    if (!(taper_of_count(c) <= taper_of_count(c + 1))) return false;
+  return true;
+}
+
 void Model::set_step_variant(int variant, int xcd_remap) {
   if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
+  if (step_is_lean(variant) && !lean_ok_)
+    throw ValueError("step variants 8 and 9 need dx and dz in [2^-40, 2^40] "
+                     "and a taper that grows towards the interior");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
   clear_graphs();  // graphs hold pointers into the injection tables
@@ -408,6 +442,12 @@ StepArgs Model::base_args() const {
   a.d4z = cfg_.dz * cfg_.dz * cfg_.dz * cfg_.dz;
   a.rdx = 1.0 / cfg_.dx;
   a.rdz = 1.0 / cfg_.dz;
+  if (lean_ok_) {
+    fast_division_constants(cfg_.dx, &a.zhx, &a.zlx);
+    fast_division_constants(cfg_.dz, &a.zhz, &a.zlz);
+  }
+  a.taper_x = taper_x_;
+  a.taper_z = taper_z_;
   a.nt = cfg_.nt;
   a.lam_k = lam_k_;
   a.mu_k = mu_k_;

@@ -23,7 +23,7 @@ def _run(model, fields, obs):
 
 
 @pytest.mark.parametrize("variant,every,nt", [(0, 4, 300), (6, 8, 295), (2, 3, 300),
-                                              (7, 5, 300)])
+                                              (7, 5, 300), (8, 4, 300)])
 def test_checkpointed_adjoint_is_bitwise_full_storage(oracle_lib, variant, every, nt):
     import psvwave_amd
     cfg = small_config(**{"domain.nt": nt})
