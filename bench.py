@@ -35,10 +35,12 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--variant", type=int, default=0,
+    p.add_argument("--variant", type=int, default=9,
                    help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split; "
                         "3-5 the same with reciprocal-multiply division; "
-                        "6, 7 two steps per launch (64x8, 64x16)")
+                        "6, 7 two steps per launch (64x8, 64x16); 8-13 lean "
+                        "one-step kernels with 64 x 16, 8, 12, 20, 24, 32 "
+                        "tiles (9 is the library default)")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
@@ -247,7 +249,8 @@ def main():
                    3: "step_fused<8,recip>", 4: "step_fused<16,recip>",
                    5: "step_split_stress<8,recip>+step_split_velocity<8,recip>",
                    6: "step_two<8,256>", 7: "step_two<16,512>",
-                   8: "step_lean<16>", 9: "step_lean<8>",
+                   8: "step_lean<16>", 9: "step_lean<8>", 10: "step_lean<12>",
+                   11: "step_lean<20>", 12: "step_lean<24>", 13: "step_lean<32>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -185,8 +185,8 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
 /* Tuning and measurement (no reference counterpart).  variant: 0 fused
  * step kernel with 64x8 tiles, 1 fused 64x16, 2 split (two launches per
  * step), 3-5 the same with reciprocal-corrected divisions, 6 / 7 two steps
- * per launch (64x8 / 64x16 tiles), 8 / 9 lean one-step kernels (64x16 /
- * 64x8 tiles; the default when dx, dz lie in [2^-40, 2^40]).  All variants
+ * per launch (64x8 / 64x16 tiles), 8-13 lean one-step kernels (64 x 16,
+ * 8, 12, 20, 24, 32 tiles; the default when dx, dz lie in [2^-40, 2^40]).  All variants
  * give bitwise identical results.  xcd_remap: 1 keeps neighbouring tiles on
  * one XCD.
  * timing_stride k > 0 brackets every k-th step launch with HIP events on
@@ -239,11 +239,11 @@ int psv_batch_get_synthetic_data(const psv_batch *b, int m, double *ux,
 int psv_batch_forward(psv_batch *b, int i_shot);
 /* L2 misfit of every model (n values), as calculate_l2_misfit. */
 int psv_batch_misfits(psv_batch *b, double *out);
-/* Step kernel for all members: 0, 1, 6, 7, 8 or 9 (see
+/* Step kernel for all members: 0, 1, 6, 7 or 8-13 (see
  * psv_set_step_variant). */
 int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
 int psv_batch_set_graph_mode(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8, 9 (ms_per_step[6],
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13 (ms_per_step[10],
  * -1 for a variant the batch cannot use) and keeps the fastest (*best).
  * Overwrites shot 0's synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,

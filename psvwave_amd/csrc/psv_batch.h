@@ -10,8 +10,8 @@
 
 namespace psv {
 
-// Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9.
-constexpr int kBatchCandidates = 6;
+// Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9, 10, 11, 12, 13.
+constexpr int kBatchCandidates = 10;
 
 class Batch {
  public:
@@ -26,7 +26,8 @@ class Batch {
   const Model &model(int m) const;
 
   // Step kernel for every member: 0 / 1 one-step 64x8 / 64x16 tiles,
-  // 6 / 7 two-step 64x8 / 64x16 tiles, 8 / 9 lean one-step 64x16 / 64x8.
+  // 6 / 7 two-step 64x8 / 64x16 tiles, 8-13 lean one-step (64 x 16, 8,
+  // 12, 20, 24, 32 tiles).
   void set_variant(int variant, int xcd_remap);
   int variant() const { return variant_; }
   void set_graph_mode(bool on);

@@ -36,7 +36,7 @@ Batch::Batch(const std::vector<Config> &cfgs) {
   for (auto &m : models_) points_ += m->grid_points();
   bool lean = true;
   for (auto &m : models_) lean = lean && m->lean_ok();
-  if (lean) variant_ = kLeanTz16;
+  if (lean) variant_ = kLeanTz8;
   build_tables();
 }
 
@@ -74,11 +74,11 @@ void Batch::set_graph_mode(bool on) {
 void Batch::set_variant(int variant, int xcd_remap) {
   if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
       variant != kTwoStepTz16 && !step_is_lean(variant))
-    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8 or 9");
+    throw ValueError("batched runs use step variants 0, 1, 6, 7 or 8-13");
   if (step_is_lean(variant))
     for (auto &m : models_)
       if (!m->lean_ok())
-        throw ValueError("a member of the batch cannot use step variants 8, 9");
+        throw ValueError("a member of the batch cannot use step variants 8-13");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
   clear_graphs();
@@ -179,8 +179,9 @@ void Batch::misfits(double *out) {
 
 double Batch::autotune(int steps, int *best, double *ms_per_variant) {
   if (steps < 2) throw ValueError("autotune needs at least 2 steps");
-  const int candidates[kBatchCandidates] = {kFusedTz8, kFusedTz16, kTwoStepTz8,
-                                            kTwoStepTz16, kLeanTz16, kLeanTz8};
+  const int candidates[kBatchCandidates] = {
+      kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16, kLeanTz16,
+      kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;

@@ -177,8 +177,8 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 
 // Step-kernel variants: 0-2 divide by the spacings with IEEE divisions,
 // 3-5 are the same kernels with RecipDiv (bit-identical results), 6-7
-// advance two time steps per launch, 8-9 are the lean one-step kernels
-// (64 x 16 and 64 x 8 tiles, FastDiv with IEEE fallback).
+// advance two time steps per launch, 8-13 are the lean one-step kernels
+// (64 x 16, 8, 12, 20, 24, 32 tiles, FastDiv with IEEE fallback).
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -190,10 +190,15 @@ enum StepVariant {
   kTwoStepTz16 = 7,
   kLeanTz16 = 8,
   kLeanTz8 = 9,
-  kNumVariants = 10
+  kLeanTz12 = 10,
+  kLeanTz20 = 11,
+  kLeanTz24 = 12,
+  kLeanTz32 = 13,
+  kNumVariants = 14
 };
+constexpr int kMaxTileHeight = 32;
 
-// Lean one-step kernels (variants 8, 9): lane = column, FastDiv, reduced
+// Lean one-step kernels (variants 8-13): lane = column, FastDiv, reduced
 // stress ring.  They need FastDiv's divisor range and a taper table that is
 // nondecreasing in the nesting count; the host checks both.
 bool step_is_lean(int variant);

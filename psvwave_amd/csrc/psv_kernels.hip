@@ -344,7 +344,7 @@ struct FastDiv {
 
 // ---------------------------------------------------------------------------
 // Lean one-step kernel.  A block of four waves owns a 64 x TZ tile; lane l of
-// wave w owns column x0 + l on rows w*R .. w*R + R - 1 (R = TZ / 4), so every
+// wave w owns column x0 + l on rows w, w+4, .., w + TZ - 4, so every
 // row a wave touches is one coalesced 512-byte segment, row offsets are
 // wave-uniform (scalar) and every LDS access is an immediate offset from one
 // per-lane address.  Global loads address a per-field scalar base plus one
@@ -371,7 +371,7 @@ struct Lean {
   static constexpr int XZR = TZ + 3, XZW = 67;  // txz: rows -2..TZ, cols -1..65
   static constexpr int NCI = 4 * TZ;        // ring-column items
   static constexpr int NCP = (NCI + 63) / 64;  // ring-column passes
-  static constexpr int KV = (VR + 3) / 4;   // v rows per wave (main columns)
+  static constexpr int KV = (TZ + 2) / 4 + 2;  // v rows per wave: w + 4m, m >= -1
   static constexpr int NSIDE = 6 * VR;      // v side items (3 + 3 columns)
   __device__ static int V(int r, int c) { return (r + 3) * VW + (c + 3); }
   __device__ static int XX(int r, int c) { return r * XXW + (c + 2); }
@@ -401,6 +401,26 @@ __device__ __forceinline__ void lean_stress(const double *sx, const double *sz,
   }
 }
 
+// Stresses of wave w's ring row: txz (w = 0), tzz and txz (w = 1, 2), tzz
+// (w = 3).
+template <int VW, class DX, class DZ>
+__device__ __forceinline__ void ring_row_stress(int w, const double *sx,
+                                                const double *sz, int v,
+                                                double lm, double mu,
+                                                double taper, double dts,
+                                                DX dx, DZ dz, double &tzz,
+                                                double &txz) {
+  double hxx = 0.0, hzz = tzz, hxz = txz;
+  if (w == 0)
+    lean_stress<kXZ, VW>(sx, sz, v, lm, mu, taper, dts, dx, dz, hxx, hzz, hxz);
+  else if (w == 3)
+    lean_stress<kZZ, VW>(sx, sz, v, lm, mu, taper, dts, dx, dz, hxx, hzz, hxz);
+  else
+    lean_stress<kZZ | kXZ, VW>(sx, sz, v, lm, mu, taper, dts, dx, dz, hxx, hzz, hxz);
+  tzz = hzz;
+  txz = hxz;
+}
+
 // Velocity at own point (r, c) from the staged stresses (velocity_point).
 template <int TZ, class DX, class DZ>
 __device__ __forceinline__ void lean_velocity(const double *sxx,
@@ -420,6 +440,32 @@ __device__ __forceinline__ void lean_velocity(const double *sxx,
   vz = taper * (vz + b * dts * az);
 }
 
+// Keeps the scheduler from interleaving the stencils of several rows (whose
+// LDS operands would then all be live at once).
+__device__ __forceinline__ void row_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// A value the compiler cannot see through: reloads through it are not
+// merged with earlier loads of the same address (whose results would then
+// stay live in registers).
+__device__ __forceinline__ unsigned opaque(unsigned v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// One 64-double row from memory straight into LDS: two transfers of one
+// dword per lane (bytes 0-255, then 256-511).  lane_off is this lane's byte
+// offset of column `lane`; lds_row (column 0 of the row) is wave-uniform.
+__device__ __forceinline__ void dma_row(const double *base, unsigned lane_off,
+                                        double *lds_row) {
+  const unsigned lane = threadIdx.x & 63;
+  const char *g = (const char *)base + (lane_off - lane * 4u);
+  __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)lds_row, 4, 0, 0);
+  __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(lds_row + 32),
+                                   4, 0, 0);
+}
+
 __device__ __forceinline__ double ldb(const double *base, unsigned off) {
   return *(const double *)((const char *)base + off);
 }
@@ -431,8 +477,14 @@ template <int TZ>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
-  __shared__ double s_vx[L::VR * L::VW], s_vz[L::VR * L::VW];
-  __shared__ double s_xx[TZ * L::XXW], s_zz[L::ZZR * 64], s_xz[L::XZR * L::XZW];
+  // One LDS buffer: velocities while the stresses are computed, then the
+  // stresses for the velocity update (a barrier separates the two uses).
+  constexpr int NV = 2 * L::VR * L::VW;
+  constexpr int NS = TZ * L::XXW + L::ZZR * 64 + L::XZR * L::XZW;
+  __shared__ double lds[NV > NS ? NV : NS];
+  double *const s_vx = lds, *const s_vz = lds + L::VR * L::VW;
+  double *const s_xx = lds, *const s_zz = lds + TZ * L::XXW;
+  double *const s_xz = s_zz + L::ZZR * 64;
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -449,15 +501,16 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
 
   // ---- phase A: every global load before the first wait ----
-  double vxm[L::KV], vzm[L::KV];
+  // Velocity rows go straight from memory into LDS (no registers): wave w
+  // stages rows r = w + 4m, m = -1 ..; its own rows are r = w + 4j,
+  // j = 0 .. R-1.
 #pragma unroll
   for (int k = 0; k < L::KV; ++k) {
-    const int r = -3 + w + 4 * k;
-    vxm[k] = vzm[k] = 0.0;
-    if (r <= TZ + 2) {
-      unsigned o = off(r, lane);
-      vxm[k] = ldb(vxi, o);
-      vzm[k] = ldb(vzi, o);
+    const int r = w + 4 * (k - 1);
+    if (r >= -3 && r <= TZ + 2) {
+      const unsigned o = off(r, lane);
+      dma_row(vxi, o, s_vx + L::V(r, 0));
+      dma_row(vzi, o, s_vz + L::V(r, 0));
     }
   }
   double vxs[2], vzs[2];
@@ -467,7 +520,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     const int t = threadIdx.x + 256 * k;
     sidx[k] = -1;
     vxs[k] = vzs[k] = 0.0;
-    if (t < L::NSIDE) {
+    if (k * 256 < L::NSIDE && t < L::NSIDE) {
       int r = t / 6 - 3, cs = t - (t / 6) * 6;
       int c = cs < 3 ? cs - 3 : 61 + cs;
       unsigned o = off(r, c);
@@ -476,11 +529,10 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
       sidx[k] = L::V(r, c);
     }
   }
-  // own stress points
   double oxx[R], ozz[R], oxz[R], olm[R], omu[R], ob[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    unsigned o = off(w * R + j, lane);
+    unsigned o = off(w + 4 * j, lane);
     oxx[j] = ldb(txxi, o);
     ozz[j] = ldb(tzzi, o);
     oxz[j] = ldb(txzi, o);
@@ -488,18 +540,19 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     omu[j] = ldb(mup, o);
     ob[j] = ldb(bp, o);
   }
-  // this wave's ring row
+  // this wave's ring row: only the fields its components read
   const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
-  double gxx, gzz, gxz, glm, gmu;
+  double gzz = 0.0, gxz = 0.0, glm = 0.0, gmu;
   {
     unsigned o = off(rr, lane);
-    gxx = ldb(txxi, o);
-    gzz = ldb(tzzi, o);
-    gxz = ldb(txzi, o);
-    glm = ldb(lmp, o);
+    if (w != 3) gxz = ldb(txzi, o);
+    if (w != 0) {
+      gzz = ldb(tzzi, o);
+      glm = ldb(lmp, o);
+    }
     gmu = ldb(mup, o);
   }
-  // ring-column items handled by this wave: item i = p * 64 + lane
+  // ring-column items of this wave: item i = p * 64 + lane
   double cxx[L::NCP], cxz[L::NCP], clm[L::NCP], cmu[L::NCP];
 #pragma unroll
   for (int p = 0; p < L::NCP; ++p) {
@@ -515,7 +568,6 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
       cmu[p] = ldb(mup, o);
     }
   }
-  // taper factors and interior masks
   const int x = x0 + lane;
   const bool xin = x < a.nx;
   const bool xint = x >= 2 && x < a.nx - 2;
@@ -523,75 +575,74 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   if (!a.adjoint && tile == 0) record_receivers(a);
 
 #pragma unroll
-  for (int k = 0; k < L::KV; ++k) {
-    const int r = -3 + w + 4 * k;
-    if (r <= TZ + 2) {
-      s_vx[L::V(r, lane)] = vxm[k];
-      s_vz[L::V(r, lane)] = vzm[k];
-    }
-  }
-#pragma unroll
   for (int k = 0; k < 2; ++k)
     if (sidx[k] >= 0) {
       s_vx[sidx[k]] = vxs[k];
       s_vz[sidx[k]] = vzs[k];
     }
+  __builtin_amdgcn_s_waitcnt(0);  // the LDS DMA of this wave has landed
   __syncthreads();
+  // start-of-step velocities of the own points
+  double vx0[R], vz0[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    vx0[j] = s_vx[L::V(w + 4 * j, lane)];
+    vz0[j] = s_vz[L::V(w + 4 * j, lane)];
+  }
 
-  // ---- phase B: stresses ----
+  // ---- phase B: stresses into registers ----
+  // The IEEE redo of a lane whose FastDiv flags fail reloads its operands
+  // from memory (rare), so only the results stay live across the phase.
   const double dts = a.dts;
-  DivFlags fl;
-  const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
   const IeeeDiv ix{a.dx}, iz{a.dz};
   auto zint = [&](int z) { return z >= 2 && z < a.nz - 2; };
   auto taper_row = [&](int z) {
     return a.taper_z[z < 0 ? 0 : (z >= a.nz ? a.nz - 1 : z)];
   };
-  double nxx[R], nzz[R], nxz[R], trow[R];
+  double trow[R];
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int r = w * R + j, z = z0 + r;
-    nxx[j] = oxx[j], nzz[j] = ozz[j], nxz[j] = oxz[j];
-    trow[j] = taper_row(z);
-    if (xint && zint(z))
-      lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), olm[j], omu[j],
-                                          fmin(tx, trow[j]), dts, fx, fz, nxx[j],
-                                          nzz[j], nxz[j]);
-  }
-  if (!fl.ok()) {
+  for (int j = 0; j < R; ++j) trow[j] = taper_row(z0 + w + 4 * j);
+  {
+    DivFlags fl;
+    const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const int r = w * R + j, z = z0 + r;
-      nxx[j] = oxx[j], nzz[j] = ozz[j], nxz[j] = oxz[j];
-      if (xint && zint(z))
+      const int r = w + 4 * j;
+      if (xint && zint(z0 + r))
         lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), olm[j], omu[j],
-                                            fmin(tx, trow[j]), dts, ix, iz, nxx[j],
-                                            nzz[j], nxz[j]);
+                                            fmin(tx, trow[j]), dts, fx, fz, oxx[j],
+                                            ozz[j], oxz[j]);
+      row_fence();
+    }
+    if (!fl.ok()) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        const unsigned o = opaque(off(r, lane));
+        oxx[j] = ldb(txxi, o), ozz[j] = ldb(tzzi, o), oxz[j] = ldb(txzi, o);
+        if (xint && zint(z0 + r))
+          lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), ldb(lmp, o),
+                                              ldb(mup, o), fmin(tx, trow[j]), dts, ix,
+                                              iz, oxx[j], ozz[j], oxz[j]);
+      }
     }
   }
-  // ring row
+  // ring row (its new values replace the loaded ones)
   {
     const int z = z0 + rr;
-    const bool in = xint && zint(z);
-    const double t = fmin(tx, taper_row(z));
-    const int v = L::V(rr, lane);
-    double hxx = gxx, hzz = gzz, hxz = gxz;
-    auto ring = [&](auto dx, auto dz) {
-      hxx = gxx, hzz = gzz, hxz = gxz;
-      if (!in) return;
-      if (w == 0)
-        lean_stress<kXZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
-      else if (w == 3)
-        lean_stress<kZZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
-      else
-        lean_stress<kZZ | kXZ, L::VW>(s_vx, s_vz, v, glm, gmu, t, dts, dx, dz, hxx, hzz, hxz);
-    };
-    DivFlags gf;
-    const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
-    ring(gx, gz);
-    if (!gf.ok()) ring(ix, iz);
-    if (w != 3) s_xz[L::XZ(rr, lane)] = hxz;
-    if (w != 0) s_zz[L::ZZ(rr, lane)] = hzz;
+    if (xint && zint(z)) {
+      const double t = fmin(tx, taper_row(z));
+      const int v = L::V(rr, lane);
+      DivFlags gf;
+      const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
+      ring_row_stress<L::VW>(w, s_vx, s_vz, v, glm, gmu, t, dts, gx, gz, gzz, gxz);
+      if (!gf.ok()) {
+        const unsigned o = opaque(off(rr, lane));
+        gzz = ldb(tzzi, o), gxz = ldb(txzi, o);
+        ring_row_stress<L::VW>(w, s_vx, s_vz, v, ldb(lmp, o), ldb(mup, o), t, dts, ix, iz,
+                               gzz, gxz);
+      }
+    }
   }
   // ring columns
 #pragma unroll
@@ -601,66 +652,88 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
       const int q = i / TZ, r = i - q * TZ;
       const int c = q < 2 ? q - 2 : 62 + q;
       const int xc = x0 + c, z = z0 + r;
-      const bool in = xc >= 2 && xc < a.nx - 2 && zint(z);
-      double hxx = cxx[p], hzz = 0.0, hxz = cxz[p];
-      if (in) {
+      if (xc >= 2 && xc < a.nx - 2 && zint(z)) {
         const double t = fmin(a.taper_x[xc], taper_row(z));
         DivFlags cf;
         const FastDiv cx{a.dx, a.rdx, a.zhx, a.zlx, &cf}, cz{a.dz, a.rdz, a.zhz, a.zlz, &cf};
+        double hxx = cxx[p], hxz = cxz[p], hzz = 0.0;
         lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), clm[p], cmu[p], t, dts, cx,
                                       cz, hxx, hzz, hxz);
         if (!cf.ok()) {
-          hxx = cxx[p], hxz = cxz[p];
-          lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), clm[p], cmu[p], t, dts,
-                                        ix, iz, hxx, hzz, hxz);
+          const unsigned o = opaque(off(r, c));
+          hxx = ldb(txxi, o), hxz = ldb(txzi, o);
+          lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), ldb(lmp, o), ldb(mup, o),
+                                        t, dts, ix, iz, hxx, hzz, hxz);
         }
+        cxx[p] = hxx, cxz[p] = hxz;
       }
-      if (q != 3) s_xx[L::XX(r, c)] = hxx;
-      if (q != 0) s_xz[L::XZ(r, c)] = hxz;
     }
   }
-  // own stresses: LDS, global stores, start-of-step snapshot work
+  // own stresses to memory; start-of-step owner work (snapshot store or
+  // kernel correlation) from the start-of-step values
   double *txxo = a.txx_o + tb, *tzzo = a.tzz_o + tb, *txzo = a.txz_o + tb;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const int r = w * R + j, z = z0 + r;
-    s_xx[L::XX(r, lane)] = nxx[j];
-    s_zz[L::ZZ(r, lane)] = nzz[j];
-    s_xz[L::XZ(r, lane)] = nxz[j];
+    const int r = w + 4 * j, z = z0 + r;
     if (xin && z < a.nz) {
-      unsigned o = off(r, lane);
-      stb(txxo, o, nxx[j]);
-      stb(tzzo, o, nzz[j]);
-      stb(txzo, o, nxz[j]);
-      if (a.do_snap)
-        owner_snapshot(a, a.snap, x, z, s_vx[L::V(r, lane)], s_vz[L::V(r, lane)],
-                       oxx[j], ozz[j], oxz[j]);
+      const unsigned o = off(r, lane);
+      if (a.do_snap) {
+        const unsigned q = opaque(o);
+        owner_snapshot(a, a.snap, x, z, vx0[j], vz0[j], ldb(txxi, q),
+                       ldb(tzzi, q), ldb(txzi, q));
+      }
+      stb(txxo, o, oxx[j]);
+      stb(tzzo, o, ozz[j]);
+      stb(txzo, o, oxz[j]);
+    }
+  }
+  __syncthreads();  // every wave is done reading the velocity stage
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r = w + 4 * j;
+    s_xx[L::XX(r, lane)] = oxx[j];
+    s_zz[L::ZZ(r, lane)] = ozz[j];
+    s_xz[L::XZ(r, lane)] = oxz[j];
+  }
+  if (w != 3) s_xz[L::XZ(rr, lane)] = gxz;
+  if (w != 0) s_zz[L::ZZ(rr, lane)] = gzz;
+#pragma unroll
+  for (int p = 0; p < L::NCP; ++p) {
+    const int i = p * 64 + lane;
+    if ((p & 3) == w && i < L::NCI) {
+      const int q = i / TZ, r = i - q * TZ;
+      const int c = q < 2 ? q - 2 : 62 + q;
+      if (q != 3) s_xx[L::XX(r, c)] = cxx[p];
+      if (q != 0) s_xz[L::XZ(r, c)] = cxz[p];
     }
   }
   __syncthreads();
 
   // ---- phase C: velocities ----
   double vx[R], vz[R];
-  DivFlags vf;
-  const FastDiv ux{a.dx, a.rdx, a.zhx, a.zlx, &vf}, uz{a.dz, a.rdz, a.zhz, a.zlz, &vf};
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int r = w * R + j, z = z0 + r;
-    vx[j] = s_vx[L::V(r, lane)];
-    vz[j] = s_vz[L::V(r, lane)];
-    if (xint && zint(z))
-      lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ux,
-                        uz, vx[j], vz[j]);
-  }
-  if (!vf.ok()) {
+  {
+    DivFlags vf;
+    const FastDiv ux{a.dx, a.rdx, a.zhx, a.zlx, &vf}, uz{a.dz, a.rdz, a.zhz, a.zlz, &vf};
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      const int r = w * R + j, z = z0 + r;
-      vx[j] = s_vx[L::V(r, lane)];
-      vz[j] = s_vz[L::V(r, lane)];
-      if (xint && zint(z))
-        lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ix,
-                          iz, vx[j], vz[j]);
+      const int r = w + 4 * j;
+      vx[j] = vx0[j];
+      vz[j] = vz0[j];
+      if (xint && zint(z0 + r))
+        lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ux,
+                          uz, vx[j], vz[j]);
+      row_fence();
+    }
+    if (!vf.ok()) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        vx[j] = vx0[j];
+        vz[j] = vz0[j];
+        if (xint && zint(z0 + r))
+          lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ix,
+                            iz, vx[j], vz[j]);
+      }
     }
   }
   // injection terms of this tile, in list order, applied by their owners
@@ -668,10 +741,10 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   for (int e = e0; e < e1; ++e) {
     const InjEntry E = a.entries[e];
     const int r = E.local / kTileX, c = E.local - r * kTileX;
-    if (r / R != w || c != lane) continue;
+    if ((r & 3) != w || c != lane) continue;
 #pragma unroll
     for (int j = 0; j < R; ++j)
-      if (j == r - w * R) {
+      if (j == (r >> 2)) {
         double term;
         if (!a.adjoint) {
           double S = a.stf[(long long)E.src * a.nt + a.it];
@@ -689,17 +762,18 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   double *vxo = a.vx_o + tb, *vzo = a.vz_o + tb;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const int z = z0 + w * R + j;
-    if (xin && z < a.nz) {
-      unsigned o = off(w * R + j, lane);
+    const int r = w + 4 * j;
+    if (xin && z0 + r < a.nz) {
+      const unsigned o = off(r, lane);
       stb(vxo, o, vx[j]);
       stb(vzo, o, vz[j]);
     }
   }
 }
 
-template <int TZ>
-__global__ void __launch_bounds__(256) step_lean(StepArgs a) {
+template <int TZ, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   lean_tile<TZ>(a, tile);
@@ -992,8 +1066,9 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
   two_step_tile<TZ, NT>(a, mt.y);
 }
 
-template <int TZ>
-__global__ void __launch_bounds__(256) batch_lean(BatchLaunch b) {
+template <int TZ, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
   if (g >= b.ntiles) return;
   const int2 mt = b.tiles[g];
@@ -1301,13 +1376,26 @@ inline unsigned blocks_for(long long n, int bs) {
 }  // namespace
 
 int step_tile_height(int variant) {
-  return (variant == kFusedTz16 || variant == kFusedTz16Recip ||
-          variant == kTwoStepTz16 || variant == kLeanTz16)
-             ? 16
-             : 8;
+  switch (variant) {
+    case kFusedTz16:
+    case kFusedTz16Recip:
+    case kTwoStepTz16:
+    case kLeanTz16:
+      return 16;
+    case kLeanTz12:
+      return 12;
+    case kLeanTz20:
+      return 20;
+    case kLeanTz24:
+      return 24;
+    case kLeanTz32:
+      return 32;
+    default:
+      return 8;
+  }
 }
 bool step_is_lean(int variant) {
-  return variant == kLeanTz16 || variant == kLeanTz8;
+  return variant >= kLeanTz16 && variant <= kLeanTz32;
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -1347,10 +1435,28 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
   dim3 g(nblocks);
   const bool tall = step_tile_height(variant) == 16;
   if (step_is_lean(variant)) {
-    if (tall)
-      hipExtLaunchKernelGGL((batch_lean<16>), g, dim3(256), 0, s, e0, e1, 0, b);
-    else
-      hipExtLaunchKernelGGL((batch_lean<8>), g, dim3(256), 0, s, e0, e1, 0, b);
+    switch (variant) {
+      case kLeanTz16:
+        hipExtLaunchKernelGGL((batch_lean<16, 3>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz8:
+        hipExtLaunchKernelGGL((batch_lean<8, 5>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz12:
+        hipExtLaunchKernelGGL((batch_lean<12, 4>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz20:
+        hipExtLaunchKernelGGL((batch_lean<20, 3>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz24:
+        hipExtLaunchKernelGGL((batch_lean<24, 2>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz32:
+        hipExtLaunchKernelGGL((batch_lean<32, 2>), g, dim3(256), 0, s, e0, e1, 0, b);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
   } else if (two) {
     if (tall)
       hipExtLaunchKernelGGL((batch_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
@@ -1405,10 +1511,22 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
       hipExtLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, nullptr, e1, 0, a);
       break;
     case kLeanTz16:
-      hipExtLaunchKernelGGL((step_lean<16>), g, b, 0, s, e0, e1, 0, a);
+      hipExtLaunchKernelGGL((step_lean<16, 3>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz8:
-      hipExtLaunchKernelGGL((step_lean<8>), g, b, 0, s, e0, e1, 0, a);
+      hipExtLaunchKernelGGL((step_lean<8, 5>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz12:
+      hipExtLaunchKernelGGL((step_lean<12, 4>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz20:
+      hipExtLaunchKernelGGL((step_lean<20, 3>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz24:
+      hipExtLaunchKernelGGL((step_lean<24, 2>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz32:
+      hipExtLaunchKernelGGL((step_lean<32, 2>), g, b, 0, s, e0, e1, 0, a);
       break;
     default:
       return hipErrorInvalidValue;

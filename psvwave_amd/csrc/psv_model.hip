@@ -37,7 +37,7 @@ std::string zero_pad(int num, int pad) {
 
 Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
   lean_ok_ = lean_eligible();
-  if (lean_ok_) variant_ = kLeanTz16;
+  if (lean_ok_) variant_ = kLeanTz8;
   allocate();
   // homogeneous background then Lame fields (src/fdModel.cpp:248-261)
   hip_check(launch_fill(vp_, ngrid_, cfg_.scalar_vp, stream_), "fill vp");
@@ -119,7 +119,9 @@ void Model::allocate() {
   // Padded stencil layout, large enough for any tile height in use.
   ntx_ = (int)((d_.nx + kTileX - 1) / kTileX);
   pitch_ = (int)round_up(kHaloX + (long long)ntx_ * kTileX + 8, 64);
-  rows_ = kHaloZ + (int)round_up(d_.nz, 16) + 8;
+  // rows down to the last tile's bottom plus the kernels' halo, for every
+  // tile height up to kMaxTileHeight
+  rows_ = kHaloZ + (int)round_up(d_.nz + kMaxTileHeight, 16) + 8;
   origin_ = (long long)kHaloZ * pitch_ + kHaloX;
   npad_ = (long long)rows_ * pitch_;
   for (int k = 0; k < 2; ++k)
@@ -347,7 +349,7 @@ void Model::set_step_variant(int variant, int xcd_remap) {
   if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
   if (step_is_lean(variant) && !lean_ok_)
-    throw ValueError("step variants 8 and 9 need dx and dz in [2^-40, 2^40] "
+    throw ValueError("step variants 8-13 need dx and dz in [2^-40, 2^40] "
                      "and a taper that grows towards the interior");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
