@@ -361,6 +361,7 @@ struct FastDiv {
 // Divisions use FastDiv; a lane whose flags fail redoes its pass with IEEE
 // divisions, so results are bitwise those of the IEEE kernels.
 enum : int { kXX = 1, kZZ = 2, kXZ = 4 };
+constexpr int kInjStage = 128;  // injection entries per tile staged in LDS
 
 template <int TZ>
 struct Lean {
@@ -473,7 +474,42 @@ __device__ __forceinline__ void stb(double *base, unsigned off, double v) {
   *(double *)((char *)base + off) = v;
 }
 
-template <int TZ>
+// Source sample of an injection entry at step a.it: the source time
+// function (forward) or this shot's adjoint source (adjoint).
+__device__ __forceinline__ double inj_sample(const StepArgs &a, const InjEntry &E) {
+  const long long k = (long long)E.src * a.nt + a.it;
+  return !a.adjoint ? a.stf[k] : (E.comp == 0 ? a.adj_x : a.adj_z)[k];
+}
+
+// Forward snapshot store (src/fdModel.cpp:555-572) of the R points
+// (x, z0w + 4j) of a lean-tile thread from its start-of-step registers.
+template <int R>
+__device__ __forceinline__ void lean_store_snapshot(const StepArgs &a, int x, int z0w,
+                                                    bool xin, const double *vx,
+                                                    const double *vz, const double *txx,
+                                                    const double *tzz,
+                                                    const double *txz) {
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int z = z0w + 4 * j;
+    if (xin && z < a.nz) {
+      const long long ci = (long long)z * a.nx + x;
+      a.snap[0][ci] = vx[j];
+      a.snap[1][ci] = vz[j];
+      a.snap[2][ci] = txx[j];
+      a.snap[3][ci] = tzz[j];
+      a.snap[4][ci] = txz[j];
+    }
+  }
+}
+
+// Byte offset of tile-relative (row r, column c) from the lean tile's field
+// bases (row -3, column -8).
+__device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
+  return (unsigned)(r + 3) * rowb + (unsigned)(c + 8) * 8u;
+}
+
+template <int TZ, bool CORR>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
@@ -493,9 +529,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   // a nonnegative 32-bit byte count.
   const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 8);
   const unsigned rowb = (unsigned)a.pitch * 8u;
-  auto off = [&](int r, int c) -> unsigned {
-    return (unsigned)(r + 3) * rowb + (unsigned)(c + 8) * 8u;
-  };
+  auto off = [&](int r, int c) -> unsigned { return lean_off(r, c, rowb); };
   const double *vxi = a.vx_i + tb, *vzi = a.vz_i + tb;
   const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
   const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
@@ -540,6 +574,26 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     omu[j] = ldb(mup, o);
     ob[j] = ldb(bp, o);
   }
+  // adjoint snapshot steps: the forward snapshot and the kernels of the own
+  // points (src/fdModel.cpp:817-861), in flight with everything else
+  bool cin[R];
+  double csv[CORR ? R : 1][5], crk[CORR ? R : 1], clk[CORR ? R : 1], cmk[CORR ? R : 1];
+  if (CORR) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int z = z0 + w + 4 * j, xx = x0 + lane;
+      cin[j] = xx < a.nx && z < a.nz && xx >= a.kx0 && xx < a.kx1 && z >= a.kz0 &&
+               z < a.kz1;
+      if (cin[j]) {
+        const long long ci = (long long)z * a.nx + xx;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) csv[j][f] = a.snap[f][ci];
+        crk[j] = a.rho_k[ci];
+        clk[j] = a.lam_k[ci];
+        cmk[j] = a.mu_k[ci];
+      }
+    }
+  }
   // this wave's ring row: only the fields its components read
   const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
   double gzz = 0.0, gxz = 0.0, glm = 0.0, gmu;
@@ -573,6 +627,18 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   const bool xint = x >= 2 && x < a.nx - 2;
   const double tx = a.taper_x[xin ? x : 0];
   if (!a.adjoint && tile == 0) record_receivers(a);
+  // This tile's injection entries and their source samples at this step,
+  // fetched by all threads at once and staged in LDS; phase C applies them
+  // in list order.
+  __shared__ double s_ival[kInjStage], s_icoef[kInjStage];
+  __shared__ int s_iloc[kInjStage];
+  const int e0 = a.tile_off[tile], ne = a.tile_off[tile + 1] - e0;
+  for (int t = threadIdx.x; t < ne && t < kInjStage; t += 256) {
+    const InjEntry E = a.entries[e0 + t];
+    s_iloc[t] = E.local | (E.comp << 16);
+    s_icoef[t] = E.coef;
+    s_ival[t] = inj_sample(a, E);
+  }
 
 #pragma unroll
   for (int k = 0; k < 2; ++k)
@@ -602,6 +668,21 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   double trow[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) trow[j] = taper_row(z0 + w + 4 * j);
+  if (CORR) {
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (cin[j]) {
+        const long long ci = (long long)(z0 + w + 4 * j) * a.nx + x;
+        correlate_point(a.wcorr, olm[j], omu[j], vx0[j], vz0[j], oxx[j], ozz[j], oxz[j],
+                        csv[j][0], csv[j][1], csv[j][2], csv[j][3], csv[j][4], crk[j],
+                        clk[j], cmk[j]);
+        a.rho_k[ci] = crk[j];
+        a.lam_k[ci] = clk[j];
+        a.mu_k[ci] = cmk[j];
+      }
+  } else if (a.do_snap && !a.adjoint) {
+    lean_store_snapshot<R>(a, x, z0 + w, xin, vx0, vz0, oxx, ozz, oxz);
+  }
   {
     DivFlags fl;
     const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
@@ -677,11 +758,6 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     const int r = w + 4 * j, z = z0 + r;
     if (xin && z < a.nz) {
       const unsigned o = off(r, lane);
-      if (a.do_snap) {
-        const unsigned q = opaque(o);
-        owner_snapshot(a, a.snap, x, z, vx0[j], vz0[j], ldb(txxi, q),
-                       ldb(tzzi, q), ldb(txzi, q));
-      }
       stb(txxo, o, oxx[j]);
       stb(tzzo, o, ozz[j]);
       stb(txzo, o, oxz[j]);
@@ -737,23 +813,25 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     }
   }
   // injection terms of this tile, in list order, applied by their owners
-  const int e0 = a.tile_off[tile], e1 = a.tile_off[tile + 1];
-  for (int e = e0; e < e1; ++e) {
-    const InjEntry E = a.entries[e];
-    const int r = E.local / kTileX, c = E.local - r * kTileX;
+  for (int e = 0; e < ne; ++e) {
+    int lc;
+    double coef, val;
+    if (e < kInjStage) {
+      lc = s_iloc[e], coef = s_icoef[e], val = s_ival[e];
+    } else {
+      const InjEntry E = a.entries[e0 + e];
+      lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E);
+    }
+    const int local = lc & 0xffff, comp = lc >> 16;
+    const int r = local / kTileX, c = local - r * kTileX;
     if ((r & 3) != w || c != lane) continue;
 #pragma unroll
     for (int j = 0; j < R; ++j)
       if (j == (r >> 2)) {
-        double term;
-        if (!a.adjoint) {
-          double S = a.stf[(long long)E.src * a.nt + a.it];
-          term = E.coef * S * a.dt * ob[j] / (E.comp == 0 ? a.d4x : a.d4z);
-        } else {
-          const double *src = E.comp == 0 ? a.adj_x : a.adj_z;
-          term = a.dt * ob[j] * src[(long long)E.src * a.nt + a.it] / a.dxdz;
-        }
-        if (E.comp == 0)
+        const double term = !a.adjoint
+                                ? coef * val * a.dt * ob[j] / (comp == 0 ? a.d4x : a.d4z)
+                                : a.dt * ob[j] * val / a.dxdz;
+        if (comp == 0)
           vx[j] += term;
         else
           vz[j] += term;
@@ -771,12 +849,14 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   }
 }
 
-template <int TZ, int WPE>
+// CORR: instantiation for adjoint snapshot steps (kernel correlation; one
+// wave per SIMD fewer, for the correlation operands in flight).
+template <int TZ, int WPE, bool CORR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ>(a, tile);
+  lean_tile<TZ, CORR>(a, tile);
 }
 
 // Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
@@ -1074,7 +1154,7 @@ batch_lean(BatchLaunch b) {
   const int2 mt = b.tiles[g];
   StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
   a.it = b.it;
-  lean_tile<TZ>(a, mt.y);
+  lean_tile<TZ, false>(a, mt.y);
 }
 
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
@@ -1488,6 +1568,7 @@ bool step_is_inplace(int variant) {
 hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
                        hipEvent_t e0, hipEvent_t e1) {
   int nblocks = a.xcd_remap ? 8 * ((a.ntiles + 7) / 8) : a.ntiles;
+  const bool corr = a.do_snap && a.adjoint;
   dim3 g(nblocks), b(kBlock), b512(512);
   switch (variant) {
     case kFusedTz8:
@@ -1511,22 +1592,40 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
       hipExtLaunchKernelGGL((step_split_velocity<8, true>), g, b, 0, s, nullptr, e1, 0, a);
       break;
     case kLeanTz16:
-      hipExtLaunchKernelGGL((step_lean<16, 3>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<16, 2, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<16, 3, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz8:
-      hipExtLaunchKernelGGL((step_lean<8, 5>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 4, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 5, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz12:
-      hipExtLaunchKernelGGL((step_lean<12, 4>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<12, 3, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<12, 4, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz20:
-      hipExtLaunchKernelGGL((step_lean<20, 3>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<20, 2, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<20, 3, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz24:
-      hipExtLaunchKernelGGL((step_lean<24, 2>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<24, 1, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<24, 2, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanTz32:
-      hipExtLaunchKernelGGL((step_lean<32, 2>), g, b, 0, s, e0, e1, 0, a);
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<32, 1, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<32, 2, false>), g, b, 0, s, e0, e1, 0, a);
       break;
     default:
       return hipErrorInvalidValue;
