@@ -10,7 +10,10 @@ with kernel correlation, kernel mapping -- 7.776e10 point-updates.
 Prints one JSON line.  ``value`` is point-updates per second of the whole
 step with all inputs resident in HBM.  ``roofline`` prices the step kernel
 (the dominant kernel) at its algorithmic bytes per launch over its average
-launch duration, measured with HIP events on the model's stream.
+time per launch: HIP events on the model's stream bracket every forward and
+adjoint time loop (the step launches and the gaps between them), divided by
+the time steps launched.  ``roofline.traffic`` is the HBM bytes per launch
+from the committed rocprofv3 counter summary for the kernel.
 ``cpu_baseline`` times the CPU oracle (OpenMP, all host cores up to 16) on a
 bounded sample of the same grid.
 """
@@ -30,6 +33,28 @@ METRIC = ("fwd+adj Gpt-updates/s and % HBM peak; FWI time-to-misfit, "
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
+def _progress(msg):
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/<kernel>_hbm_traffic.md, line `bytes_per_launch = N`),
+    or None when no summary for this kernel exists."""
+    name = kernel.replace("<", "").replace(">", "").replace(",", "_")
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{name}_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("bytes_per_launch ="):
+                    return float(line.split("=", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def _args():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -44,8 +69,10 @@ def _args():
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
-    p.add_argument("--timing-stride", type=int, default=7,
-                   help="bracket every k-th step launch with HIP events")
+    p.add_argument("--timing-stride", type=int, default=-1,
+                   help="-1: HIP events around every time loop (average per "
+                        "time step, gaps included); k > 0: kernel start/end "
+                        "of every k-th step launch")
     p.add_argument("--nt", type=int, default=6000)
     p.add_argument("--shots", type=int, default=8)
     p.add_argument("--cpu-baseline-nt", type=int, default=4000,
@@ -139,12 +166,15 @@ def monte_carlo(a):
         batch.forward_all()
         return batch.misfits()
 
-    for _ in range(a.warmup):
+    _progress("setup done")
+    for k in range(a.warmup):
         step()
+        _progress(f"warmup step {k + 1}/{a.warmup}")
     batch.timing(reset=True)
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
         mis = step()
+        _progress(f"timed step {k + 1}/{a.steps}")
     secs = time.perf_counter() - t0
     loop_ms, points = batch.timing()
     nt = batch.nt
@@ -222,13 +252,16 @@ def main():
             model.adjoint_simulate(s)
         model.map_kernels_to_velocity()
 
-    for _ in range(a.warmup):
+    _progress("setup done")
+    for k in range(a.warmup):
         step()
+        _progress(f"warmup step {k + 1}/{a.warmup}")
     model.set_timing(a.timing_stride)
     model.reset_step_timing()
     t0 = time.perf_counter()
-    for _ in range(a.steps):
+    for k in range(a.steps):
         step()  # every call synchronises the model stream before returning
+        _progress(f"timed step {k + 1}/{a.steps}")
     secs = time.perf_counter() - t0
     kern_ms, timed, launches = model.step_timing()
     model.set_timing(0)
@@ -284,12 +317,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": _measured_traffic(kernel_name),
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_us": round(avg_launch_s * 1e6, 3),
             "timed_launches": timed,
             "launches": launches,
             "launches_per_time_step": launches_per_kernel,
+            "timing": ("HIP events around each time loop, per time step, gaps "
+                       "included" if a.timing_stride < 0 else
+                       f"kernel start/end of every {a.timing_stride}th launch"),
         },
     }
     if copy:

@@ -186,12 +186,14 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * step kernel with 64x8 tiles, 1 fused 64x16, 2 split (two launches per
  * step), 3-5 the same with reciprocal-corrected divisions, 6 / 7 two steps
  * per launch (64x8 / 64x16 tiles), 8-13 lean one-step kernels (64 x 16,
- * 8, 12, 20, 24, 32 tiles; the default when dx, dz lie in [2^-40, 2^40]).  All variants
- * give bitwise identical results.  xcd_remap: 1 keeps neighbouring tiles on
- * one XCD.
- * timing_stride k > 0 brackets every k-th step launch with HIP events on
- * the model's stream; psv_get_step_timing returns the summed milliseconds
- * and the number of bracketed launches since the last reset. */
+ * 8, 12, 20, 24, 32 tiles; the default when dx, dz lie in [2^-40, 2^40]).
+ * All variants give bitwise identical results.  xcd_remap: 1 keeps
+ * neighbouring tiles on one XCD.
+ * timing_stride k > 0 records the kernel start and end of every k-th step
+ * launch; k < 0 brackets whole time loops with HIP events on the model's
+ * stream (launches plus the gaps between them); psv_get_step_timing returns
+ * the summed milliseconds and the number of time steps covered since the
+ * last reset. */
 int psv_set_step_variant(psv_model *m, int variant, int xcd_remap);
 int psv_set_timing(psv_model *m, int timing_stride);
 /* on = 1: capture each (shot, direction, store_fields) time loop once as a

@@ -53,13 +53,13 @@ __device__ inline int block_tile(const StepArgs &a) {
 
 // Receiver record at the start of step it (src/fdModel.cpp:576-596):
 // running sum of dt * v / (dx * dz).
-__device__ inline void record_receivers(const StepArgs &a) {
+__device__ inline void record_receivers(const StepArgs &a, int it) {
   for (int r = threadIdx.x; r < a.nr; r += blockDim.x) {
     long long li = a.rec_idx[r];
     double incx = a.dt * a.vx_i[li] / a.dxdz;
     double incz = a.dt * a.vz_i[li] / a.dxdz;
-    long long k = (long long)r * a.nt + a.it;
-    if (a.it == 0 || a.rec_increments) {
+    long long k = (long long)r * a.nt + it;
+    if (it == 0 || a.rec_increments) {
       a.rtf_x[k] = incx;
       a.rtf_z[k] = incz;
     } else {
@@ -234,7 +234,7 @@ __device__ __forceinline__ void fused_tile(const StepArgs &a, const int tile) {
       s_txz[i] = txz_r[k];
     }
   }
-  if (!a.adjoint && tile == 0) record_receivers(a);
+  if (!a.adjoint && tile == 0) record_receivers(a, a.it);
   __syncthreads();
 
   // phase B: stresses on the tile plus a 2-point ring, in place in LDS (a
@@ -476,9 +476,11 @@ __device__ __forceinline__ void stb(double *base, unsigned off, double v) {
 
 // Source sample of an injection entry at step a.it: the source time
 // function (forward) or this shot's adjoint source (adjoint).
-__device__ __forceinline__ double inj_sample(const StepArgs &a, const InjEntry &E) {
-  const long long k = (long long)E.src * a.nt + a.it;
-  return !a.adjoint ? a.stf[k] : (E.comp == 0 ? a.adj_x : a.adj_z)[k];
+__device__ __forceinline__ double inj_sample(const StepArgs &a, const InjEntry &E,
+                                            int it) {
+  const long long k = (long long)E.src * a.nt + it;
+  const double *ax = a.adj_x, *az = a.adj_z;  // select values, not members
+  return !a.adjoint ? a.stf[k] : (E.comp == 0 ? ax : az)[k];
 }
 
 // Forward snapshot store (src/fdModel.cpp:555-572) of the R points
@@ -510,7 +512,7 @@ __device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
 }
 
 template <int TZ, bool CORR>
-__device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
+__device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, const int it) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
   // One LDS buffer: velocities while the stresses are computed, then the
@@ -626,7 +628,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
   const bool xin = x < a.nx;
   const bool xint = x >= 2 && x < a.nx - 2;
   const double tx = a.taper_x[xin ? x : 0];
-  if (!a.adjoint && tile == 0) record_receivers(a);
+  if (!a.adjoint && tile == 0) record_receivers(a, it);
   // This tile's injection entries and their source samples at this step,
   // fetched by all threads at once and staged in LDS; phase C applies them
   // in list order.
@@ -637,7 +639,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
     const InjEntry E = a.entries[e0 + t];
     s_iloc[t] = E.local | (E.comp << 16);
     s_icoef[t] = E.coef;
-    s_ival[t] = inj_sample(a, E);
+    s_ival[t] = inj_sample(a, E, it);
   }
 
 #pragma unroll
@@ -820,7 +822,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile) {
       lc = s_iloc[e], coef = s_icoef[e], val = s_ival[e];
     } else {
       const InjEntry E = a.entries[e0 + e];
-      lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E);
+      lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E, it);
     }
     const int local = lc & 0xffff, comp = lc >> 16;
     const int r = local / kTileX, c = local - r * kTileX;
@@ -856,7 +858,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ, CORR>(a, tile);
+  lean_tile<TZ, CORR>(a, tile, a.it);
 }
 
 // Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
@@ -1152,9 +1154,8 @@ batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
   if (g >= b.ntiles) return;
   const int2 mt = b.tiles[g];
-  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
-  a.it = b.it;
-  lean_tile<TZ, false>(a, mt.y);
+  const StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  lean_tile<TZ, false>(a, mt.y, b.it);
 }
 
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
@@ -1178,7 +1179,7 @@ __global__ void __launch_bounds__(kBlock) step_split_stress(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
-  if (!a.adjoint && tile == 0) record_receivers(a);
+  if (!a.adjoint && tile == 0) record_receivers(a, a.it);
   for (int i = threadIdx.x; i < TZ * kTileX; i += kBlock) {
     int r = i / kTileX, c = i - r * kTileX;
     int x = x0 + c, z = z0 + r;

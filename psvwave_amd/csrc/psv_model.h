@@ -66,6 +66,9 @@ class Model {
 
   // tuning and measurement
   void set_step_variant(int variant, int xcd_remap);
+  // stride k > 0: HIP start/stop timestamps of every k-th step launch;
+  // k < 0: events bracketing whole time loops (launches and the gaps
+  // between them); 0: off.
   void set_timing(int stride) { timing_stride_ = stride; }
   // Capture each (direction, shot, store) time loop once as a HIP graph and
   // replay it; step timing then brackets whole loops.
@@ -149,6 +152,7 @@ class Model {
   bool graph_mode_ = false;
   std::map<long long, hipGraphExec_t> graphs_;
   void clear_graphs();
+  void timed_loop(const std::function<void()> &enqueue);
   using SlotFn = std::function<bool(int, double **)>;
   int enqueue_range(StepArgs &a, double *(*st)[5], int parity, bool forward,
                     int k0, int k1, bool snaps, const SlotFn &slot,

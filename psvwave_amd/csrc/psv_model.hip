@@ -689,7 +689,7 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
       hip_check(e, "hipGraphInstantiate");
       it = graphs_.emplace(key, exec).first;
     }
-    bool timed = timing_stride_ > 0;
+    bool timed = timing_stride_ != 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
       while (events_.size() < 2) {
@@ -713,6 +713,10 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
     }
     return;
   }
+  if (timing_stride_ < 0) {  // bracket the whole loop
+    timed_loop([&] { enqueue_loop(a, i_shot, forward, store, nullptr, dump); });
+    return;
+  }
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
   enqueue_loop(a, i_shot, forward, store, &pairs, dump);
   sync();
@@ -722,6 +726,26 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
               "hipEventElapsedTime");
     timed_ms_ += ms;
   }
+}
+
+// Loop timing: HIP events recorded on the model stream before and after the
+// enqueued launches; the bracket covers every launch of the loop and the gaps
+// between them, and counts the time steps those launches advanced.
+void Model::timed_loop(const std::function<void()> &enqueue) {
+  while (events_.size() < 2) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    events_.push_back(e);
+  }
+  const int64_t before = launches_;
+  hip_check(hipEventRecord(events_[0], stream_), "hipEventRecord");
+  enqueue();
+  hip_check(hipEventRecord(events_[1], stream_), "hipEventRecord");
+  sync();
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, events_[0], events_[1]), "hipEventElapsedTime");
+  timed_ms_ += ms;
+  timed_ += launches_ - before;
 }
 
 StepArgs Model::forward_launch_args(int i_shot, int parity) const {
@@ -811,7 +835,9 @@ void Model::adjoint(int i_shot, bool verbose) {
   a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
   auto t0 = std::chrono::steady_clock::now();
-  if (ckpt_every_ > 1) {
+  if (ckpt_every_ > 1 && timing_stride_ < 0) {
+    timed_loop([&] { enqueue_checkpointed_adjoint(a, i_shot, nullptr); });
+  } else if (ckpt_every_ > 1) {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
     enqueue_checkpointed_adjoint(a, i_shot, &pairs);
     sync();

@@ -387,7 +387,9 @@ class fdModel:
         _lib.check(self._lib.psv_set_checkpointing(self._h, int(every)))
 
     def set_timing(self, stride):
-        """Bracket every `stride`-th step launch with HIP events (0: off)."""
+        """Step timing: stride k > 0 records the kernel start and end of every
+        k-th step launch; k < 0 brackets whole time loops with HIP events
+        (launches plus gaps); 0 turns timing off."""
         _lib.check(self._lib.psv_set_timing(self._h, int(stride)))
 
     def step_timing(self):
