@@ -534,8 +534,7 @@ int psv_set_checkpointing(psv_model *m, int every) {
 
 int psv_set_timing(psv_model *m, int stride) {
   return guard([&] {
-    if (stride < 0) throw psv::ValueError("timing stride must be >= 0");
-    M(m).set_timing(stride);
+    M(m).set_timing(stride < 0 ? -1 : stride);
   });
 }
 

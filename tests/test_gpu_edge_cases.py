@@ -291,3 +291,28 @@ def test_graph_replays_match_eager_mode(oracle_lib):
         for a, b in zip(se + ke, sg + kg):
             assert np.array_equal(a, b), f"replay {rep} differs"
         assert me == mg
+
+
+def test_step_timing_modes_leave_results_unchanged(oracle_lib):
+    """Kernel-timestamp timing (stride > 0), loop bracketing (stride < 0) and
+    no timing give the same seismograms; the counters cover the time steps."""
+    import psvwave_amd
+    cfg = small_config(**{"domain.nt": 120})
+    out = []
+    for stride in (0, 7, -1):
+        m = psvwave_amd.fdModel(cfg)
+        m.set_timing(stride)
+        m.reset_step_timing()
+        for s in range(m.n_shots):
+            m.forward_simulate(s, store_fields=False)
+        ms, timed, launches = m.step_timing()
+        assert launches == m.n_shots * 120
+        if stride == 0:
+            assert timed == 0
+        elif stride < 0:
+            assert timed == launches and ms > 0
+        else:
+            assert 0 < timed < launches and ms > 0
+        out.append(m.get_synthetic_data())
+    for a, b in zip(out[1:], out[:-1]):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
