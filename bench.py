@@ -65,7 +65,8 @@ def _args():
                         "3-5 the same with reciprocal-multiply division; "
                         "6, 7 two steps per launch (64x8, 64x16); 8-13 lean "
                         "one-step kernels with 64 x 16, 8, 12, 20, 24, 32 "
-                        "tiles (9 is the library default)")
+                        "tiles (9 is the library default); 14 lean two "
+                        "steps per launch (64x8)")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
@@ -274,7 +275,7 @@ def main():
         (d["nz_inner"] - 2 * d["nz_inner_boundary"])
     bytes_per_launch = algorithmic_bytes_per_shot(
         nx, nz, nt, cfg["inversion"]["snapshot_interval"], kpts) / (2.0 * nt)
-    launches_per_kernel = {2: 2, 5: 2, 6: 0.5, 7: 0.5}.get(a.variant, 1)
+    launches_per_kernel = {2: 2, 5: 2, 6: 0.5, 7: 0.5, 14: 0.5}.get(a.variant, 1)
     avg_launch_s = kern_ms / 1e3 / max(timed, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
     kernel_name = {0: "step_fused<8,ieee>", 1: "step_fused<16,ieee>",
@@ -284,6 +285,7 @@ def main():
                    6: "step_two<8,256>", 7: "step_two<16,512>",
                    8: "step_lean<16>", 9: "step_lean<8>", 10: "step_lean<12>",
                    11: "step_lean<20>", 12: "step_lean<24>", 13: "step_lean<32>",
+                   14: "step_lean2<8>",
                    }[a.variant]
     out = {
         "metric": METRIC,

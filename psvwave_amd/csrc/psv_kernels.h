@@ -178,7 +178,8 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 // Step-kernel variants: 0-2 divide by the spacings with IEEE divisions,
 // 3-5 are the same kernels with RecipDiv (bit-identical results), 6-7
 // advance two time steps per launch, 8-13 are the lean one-step kernels
-// (64 x 16, 8, 12, 20, 24, 32 tiles, FastDiv with IEEE fallback).
+// (64 x 16, 8, 12, 20, 24, 32 tiles, FastDiv with IEEE fallback), 14 the
+// lean two-step kernel (64 x 8 tiles).
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -194,14 +195,16 @@ enum StepVariant {
   kLeanTz20 = 11,
   kLeanTz24 = 12,
   kLeanTz32 = 13,
-  kNumVariants = 14
+  kLeanTwoTz8 = 14,
+  kNumVariants = 15
 };
 constexpr int kMaxTileHeight = 32;
 
 // Lean one-step kernels (variants 8-13): lane = column, FastDiv, reduced
 // stress ring.  They need FastDiv's divisor range and a taper table that is
 // nondecreasing in the nesting count; the host checks both.
-bool step_is_lean(int variant);
+bool step_is_lean(int variant);       // lean one-step kernels (8-13)
+bool step_uses_fastdiv(int variant);  // lean one- and two-step kernels
 bool fast_division_ok(double d);
 void fast_division_constants(double d, double *zh, double *zl);
 

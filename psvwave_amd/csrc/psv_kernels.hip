@@ -861,6 +861,486 @@ step_lean(StepArgs a) {
   lean_tile<TZ, CORR>(a, tile, a.it);
 }
 
+// ---------------------------------------------------------------------------
+// Lean two-step kernel: steps s0 = a.it and s1 = a.it1 of one 64 x TZ tile
+// per launch (temporal blocking: the state crosses HBM once per two steps).
+// Four LDS stages updated in place, with the tile-relative boxes
+//   v0  rows -8 .. TZ+7, cols -8 .. 71   (A, loaded)
+//   σ1  rows -6 .. TZ+5, cols -6 .. 69   (B: σ0 loaded, updated in place)
+//   v1  rows -4 .. TZ+3, cols -4 .. 67   (A, in place; step s0's injection
+//                                          terms on the whole box)
+//   σ2  rows -2 .. TZ+1, cols -2 .. 65   (B, in place)
+//   v2  rows  0 .. TZ-1, cols  0 .. 63   (to memory; step s1's injection)
+// Every stage maps lane l of wave w to column l on rows w + 4k of its box,
+// plus one pass of "side" items (the box columns outside 0 .. 63, lane ->
+// (column, row)).  An update reads only neighbours from the other buffer and
+// its own old value, so the in-place writes need no extra barrier.  Owners
+// of the tile's points record receivers at both steps (increments; a scan
+// forms the running sums after the loop), store forward snapshots and
+// correlate adjoint ones from the start-of-step values in LDS.
+template <int TZ>
+struct Lean2 {
+  static constexpr int AH = TZ + 16, AW = 80;  // v box
+  static constexpr int BH = TZ + 12, BW = 76;  // σ box
+  static constexpr int KA = AH / 4;            // A rows per wave
+  static constexpr int K1 = BH / 4;            // σ1 rows per wave
+  static constexpr int KV = (TZ + 8) / 4;      // v1 rows per wave
+  static constexpr int NSA = 16 * AH;          // A side items (16 columns)
+  static constexpr int NS1 = 12 * BH;          // σ1 side items (12 columns)
+  static constexpr int NV1 = 8 * (TZ + 8);     // v1 side items (8 columns)
+  static constexpr int NS2 = 4 * (TZ + 4);     // σ2 side items (4 columns)
+  __device__ static int A(int r, int c) { return (r + 8) * AW + (c + 8); }
+  __device__ static int B(int r, int c) { return (r + 6) * BW + (c + 6); }
+  // side columns of a box of half-width h (h columns left of 0, h right of 63)
+  __device__ static int side_col(int s, int h) { return s < h ? s - h : 64 + (s - h); }
+};
+
+// Velocity from stresses in a box of width W (velocity_point's trees).
+template <int W, class DX, class DZ>
+__device__ __forceinline__ void l2_velocity(const double *sxx, const double *szz,
+                                            const double *sxz, int i, double b,
+                                            double taper, double dts, DX divx,
+                                            DZ divz, double &vx, double &vz) {
+  double ax = divx(kC1 * (sxx[i] - sxx[i - 1]) + kC2 * (sxx[i - 2] - sxx[i + 1])) +
+              divz(kC1 * (sxz[i] - sxz[i - W]) + kC2 * (sxz[i - 2 * W] - sxz[i + W]));
+  double az = divx(kC1 * (sxz[i + 1] - sxz[i]) + kC2 * (sxz[i - 1] - sxz[i + 2])) +
+              divz(kC1 * (szz[i + W] - szz[i]) + kC2 * (szz[i - W] - szz[i + 2 * W]));
+  vx = taper * (vx + b * dts * ax);
+  vz = taper * (vz + b * dts * az);
+}
+
+template <int TZ>
+__device__ __forceinline__ void lean2_tile(const StepArgs &a, const int tile) {
+  using L = Lean2<TZ>;
+  constexpr int R = TZ / 4;
+  __shared__ double s_vx[L::AH * L::AW], s_vz[L::AH * L::AW];
+  __shared__ double s_xx[L::BH * L::BW], s_zz[L::BH * L::BW], s_xz[L::BH * L::BW];
+  __shared__ double s_ival[2][kInjStage], s_icoef[2][kInjStage];
+  __shared__ int s_iloc[2][kInjStage];
+
+  const int lane = threadIdx.x & 63, tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  // field bases at tile-relative (row -8, column -8)
+  const long long tb = a.origin + (long long)(z0 - 8) * a.pitch + (x0 - 8);
+  const unsigned rowb = (unsigned)a.pitch * 8u;
+  auto off = [&](int r, int c) -> unsigned {
+    return (unsigned)(r + 8) * rowb + (unsigned)(c + 8) * 8u;
+  };
+  const double *vxi = a.vx_i + tb, *vzi = a.vz_i + tb;
+  const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
+  const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
+  auto interior = [&](int r, int c) {
+    const int x = x0 + c, z = z0 + r;
+    return x >= 2 && x < a.nx - 2 && z >= 2 && z < a.nz - 2;
+  };
+  auto taper = [&](int r, int c) {
+    int x = x0 + c, z = z0 + r;
+    x = x < 0 ? 0 : (x >= a.nx ? a.nx - 1 : x);
+    z = z < 0 ? 0 : (z >= a.nz ? a.nz - 1 : z);
+    return fmin(a.taper_x[x], a.taper_z[z]);
+  };
+
+  // ---- phase A: every load before the first wait ----
+  // main columns 0..63 of A and B straight into LDS
+#pragma unroll
+  for (int k = 0; k < L::KA; ++k) {
+    const int r = -8 + w + 4 * k;
+    const unsigned o = off(r, lane);
+    dma_row(vxi, o, s_vx + L::A(r, 0));
+    dma_row(vzi, o, s_vz + L::A(r, 0));
+  }
+#pragma unroll
+  for (int k = 0; k < L::K1; ++k) {
+    const int r = -6 + w + 4 * k;
+    const unsigned o = off(r, lane);
+    dma_row(txxi, o, s_xx + L::B(r, 0));
+    dma_row(tzzi, o, s_zz + L::B(r, 0));
+    dma_row(txzi, o, s_xz + L::B(r, 0));
+  }
+  // side columns through registers
+  double avx[2], avz[2];
+  int aidx[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int t = tid + 256 * k;
+    aidx[k] = -1;
+    avx[k] = avz[k] = 0.0;
+    if (k * 256 < L::NSA && t < L::NSA) {
+      const int s = t / L::AH, r = t - s * L::AH - 8, c = L::side_col(s, 8);
+      const unsigned o = off(r, c);
+      avx[k] = ldb(vxi, o);
+      avz[k] = ldb(vzi, o);
+      aidx[k] = L::A(r, c);
+    }
+  }
+  double bxx = 0.0, bzz = 0.0, bxz = 0.0, slm = 0.0, smu = 0.0;
+  const bool has1 = tid < L::NS1;
+  const int s1r = has1 ? tid % L::BH - 6 : 0, s1c = has1 ? L::side_col(tid / L::BH, 6) : 0;
+  if (has1) {
+    const unsigned o = off(s1r, s1c);
+    bxx = ldb(txxi, o), bzz = ldb(tzzi, o), bxz = ldb(txzi, o);
+    slm = ldb(lmp, o), smu = ldb(mup, o);
+  }
+  // materials of this thread's main items
+  double lm1[L::K1], mu1[L::K1], bv[L::KV];
+#pragma unroll
+  for (int k = 0; k < L::K1; ++k) {
+    const unsigned o = off(-6 + w + 4 * k, lane);
+    lm1[k] = ldb(lmp, o);
+    mu1[k] = ldb(mup, o);
+  }
+#pragma unroll
+  for (int k = 0; k < L::KV; ++k) bv[k] = ldb(bp, off(-4 + w + 4 * k, lane));
+  const bool hasv = tid < L::NV1;
+  const int v1r = hasv ? tid % (TZ + 8) - 4 : 0, v1c = hasv ? L::side_col(tid / (TZ + 8), 4) : 0;
+  const double vb = hasv ? ldb(bp, off(v1r, v1c)) : 0.0;
+  const bool has2 = tid < L::NS2;
+  const int s2r = has2 ? tid % (TZ + 4) - 2 : 0, s2c = has2 ? L::side_col(tid / (TZ + 4), 2) : 0;
+  double s2lm = 0.0, s2mu = 0.0;
+  if (has2) {
+    const unsigned o = off(s2r, s2c);
+    s2lm = ldb(lmp, o), s2mu = ldb(mup, o);
+  }
+  // injection entries: step s0 on the v1 box (halo table, local index over
+  // (64 + 8) x (TZ + 8) from (x0 - 4, z0 - 4)), step s1 on the tile
+  const int h0 = a.halo_off[tile], nh = a.halo_off[tile + 1] - h0;
+  const int e0 = a.tile_off[tile], ne = a.tile_off[tile + 1] - e0;
+  for (int t = tid; t < nh && t < kInjStage; t += 256) {
+    const InjEntry E = a.halo_entries[h0 + t];
+    s_iloc[0][t] = E.local | (E.comp << 16);
+    s_icoef[0][t] = E.coef;
+    s_ival[0][t] = inj_sample(a, E, a.it);
+  }
+  for (int t = tid; t < ne && t < kInjStage; t += 256) {
+    const InjEntry E = a.entries[e0 + t];
+    s_iloc[1][t] = E.local | (E.comp << 16);
+    s_icoef[1][t] = E.coef;
+    s_ival[1][t] = inj_sample(a, E, a.it1);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (aidx[k] >= 0) {
+      s_vx[aidx[k]] = avx[k];
+      s_vz[aidx[k]] = avz[k];
+    }
+  if (has1) {
+    const int i = L::B(s1r, s1c);
+    s_xx[i] = bxx, s_zz[i] = bzz, s_xz[i] = bxz;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  const double dts = a.dts;
+  const IeeeDiv ix{a.dx}, iz{a.dz};
+  const int xo = x0 + lane;  // this lane's column in main items
+  const bool xin = xo < a.nx;
+  // owner work at the start of a step for own point (r, lane): forward
+  // snapshot store or adjoint correlation (src/fdModel.cpp:555-572, 817-861)
+  auto owner_start = [&](double *const *snap, bool do_snap, int r) {
+    const int z = z0 + r;
+    if (!do_snap || !xin || z >= a.nz) return;
+    const int ia = L::A(r, lane), ib = L::B(r, lane);
+    owner_snapshot(a, snap, xo, z, s_vx[ia], s_vz[ia], s_xx[ib], s_zz[ib], s_xz[ib]);
+  };
+
+  // ---- stage 1: σ1 on the σ box ----
+  {
+    double n1[L::K1][3];
+    DivFlags fl;
+    const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+#pragma unroll
+    for (int k = 0; k < L::K1; ++k) {
+      const int r = -6 + w + 4 * k, ib = L::B(r, lane);
+      n1[k][0] = s_xx[ib], n1[k][1] = s_zz[ib], n1[k][2] = s_xz[ib];
+      if (r >= 0 && r < TZ) owner_start(a.snap, a.do_snap, r);
+      if (interior(r, lane))
+        lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(r, lane), lm1[k], mu1[k],
+                                            taper(r, lane), dts, fx, fz, n1[k][0],
+                                            n1[k][1], n1[k][2]);
+      row_fence();
+    }
+    if (!fl.ok()) {
+#pragma unroll
+      for (int k = 0; k < L::K1; ++k) {
+        const int r = -6 + w + 4 * k, ib = L::B(r, lane);
+        n1[k][0] = s_xx[ib], n1[k][1] = s_zz[ib], n1[k][2] = s_xz[ib];
+        if (interior(r, lane))
+          lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(r, lane), lm1[k], mu1[k],
+                                              taper(r, lane), dts, ix, iz, n1[k][0],
+                                              n1[k][1], n1[k][2]);
+      }
+    }
+    double h[3] = {bxx, bzz, bxz};
+    if (has1 && interior(s1r, s1c)) {
+      DivFlags sf;
+      const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &sf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &sf};
+      const double t = taper(s1r, s1c);
+      lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(s1r, s1c), slm, smu, t, dts, gx,
+                                          gz, h[0], h[1], h[2]);
+      if (!sf.ok()) {
+        h[0] = bxx, h[1] = bzz, h[2] = bxz;
+        lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(s1r, s1c), slm, smu, t, dts,
+                                            ix, iz, h[0], h[1], h[2]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < L::K1; ++k) {
+      const int ib = L::B(-6 + w + 4 * k, lane);
+      s_xx[ib] = n1[k][0], s_zz[ib] = n1[k][1], s_xz[ib] = n1[k][2];
+    }
+    if (has1) {
+      const int ib = L::B(s1r, s1c);
+      s_xx[ib] = h[0], s_zz[ib] = h[1], s_xz[ib] = h[2];
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 2: v1 on the v box, step s0's injection, receivers ----
+  {
+    double vx[L::KV], vz[L::KV];
+    DivFlags fl;
+    const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+#pragma unroll
+    for (int k = 0; k < L::KV; ++k) {
+      const int r = -4 + w + 4 * k, ia = L::A(r, lane);
+      vx[k] = s_vx[ia], vz[k] = s_vz[ia];
+      if (interior(r, lane))
+        l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(r, lane), bv[k], taper(r, lane), dts,
+                           fx, fz, vx[k], vz[k]);
+      row_fence();
+    }
+    if (!fl.ok()) {
+#pragma unroll
+      for (int k = 0; k < L::KV; ++k) {
+        const int r = -4 + w + 4 * k, ia = L::A(r, lane);
+        vx[k] = s_vx[ia], vz[k] = s_vz[ia];
+        if (interior(r, lane))
+          l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(r, lane), bv[k], taper(r, lane), dts,
+                             ix, iz, vx[k], vz[k]);
+      }
+    }
+    double svx = 0.0, svz = 0.0;
+    if (hasv) {
+      const int ia = L::A(v1r, v1c);
+      svx = s_vx[ia], svz = s_vz[ia];
+      if (interior(v1r, v1c)) {
+        DivFlags sf;
+        const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &sf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &sf};
+        const double t = taper(v1r, v1c);
+        l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(v1r, v1c), vb, t, dts, gx, gz, svx, svz);
+        if (!sf.ok()) {
+          svx = s_vx[ia], svz = s_vz[ia];
+          l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(v1r, v1c), vb, t, dts, ix, iz, svx,
+                             svz);
+        }
+      }
+    }
+    // step s0's injection terms on the v box, in list order, by owners
+    for (int e = 0; e < nh; ++e) {
+      int lc;
+      double coef, val;
+      if (e < kInjStage) {
+        lc = s_iloc[0][e], coef = s_icoef[0][e], val = s_ival[0][e];
+      } else {
+        const InjEntry E = a.halo_entries[h0 + e];
+        lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E, a.it);
+      }
+      const int local = lc & 0xffff, comp = lc >> 16;
+      const int r = local / (kTileX + 8) - 4, c = local % (kTileX + 8) - 4;
+      if (c >= 0 && c < 64) {
+        if (c != lane || ((r + 4) & 3) != w) continue;
+#pragma unroll
+        for (int k = 0; k < L::KV; ++k)
+          if (k == ((r + 4) >> 2)) {
+            const double term = !a.adjoint
+                                    ? coef * val * a.dt * bv[k] / (comp == 0 ? a.d4x : a.d4z)
+                                    : a.dt * bv[k] * val / a.dxdz;
+            if (comp == 0)
+              vx[k] += term;
+            else
+              vz[k] += term;
+          }
+      } else {
+        if (!hasv || r != v1r || c != v1c) continue;
+        const double term = !a.adjoint ? coef * val * a.dt * vb / (comp == 0 ? a.d4x : a.d4z)
+                                       : a.dt * vb * val / a.dxdz;
+        if (comp == 0)
+          svx += term;
+        else
+          svz += term;
+      }
+    }
+    // owners: receivers at the start of s0 and s1, forward snapshot of v1
+#pragma unroll
+    for (int k = 0; k < L::KV; ++k) {
+      const int r = -4 + w + 4 * k;
+      if (r < 0 || r >= TZ || !xin || z0 + r >= a.nz) continue;
+      if (!a.adjoint) {
+        const int ia = L::A(r, lane), local = r * kTileX + lane;
+        record_owned(a, tile, local, a.it, s_vx[ia], s_vz[ia]);
+        record_owned(a, tile, local, a.it1, vx[k], vz[k]);
+        if (a.do_snap1) {
+          const long long ci = (long long)(z0 + r) * a.nx + xo;
+          a.snap1[0][ci] = vx[k];
+          a.snap1[1][ci] = vz[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < L::KV; ++k) {
+      const int ia = L::A(-4 + w + 4 * k, lane);
+      s_vx[ia] = vx[k], s_vz[ia] = vz[k];
+    }
+    if (hasv) {
+      const int ia = L::A(v1r, v1c);
+      s_vx[ia] = svx, s_vz[ia] = svz;
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 3: σ2 on the σ2 box (main rows are stage-1 rows k = 1..3) ----
+  double *txxo = a.txx_o + tb, *tzzo = a.tzz_o + tb, *txzo = a.txz_o + tb;
+  {
+    constexpr int K2 = (TZ + 4) / 4;
+    double n2[K2][3];
+    DivFlags fl;
+    const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+#pragma unroll
+    for (int k = 0; k < K2; ++k) {
+      const int r = -2 + w + 4 * k, ib = L::B(r, lane);
+      n2[k][0] = s_xx[ib], n2[k][1] = s_zz[ib], n2[k][2] = s_xz[ib];
+      if (r >= 0 && r < TZ) {
+        if (a.adjoint) {
+          owner_start(a.snap1, a.do_snap1, r);
+        } else if (a.do_snap1 && xin && z0 + r < a.nz) {
+          const long long ci = (long long)(z0 + r) * a.nx + xo;
+          a.snap1[2][ci] = n2[k][0];
+          a.snap1[3][ci] = n2[k][1];
+          a.snap1[4][ci] = n2[k][2];
+        }
+      }
+      if (interior(r, lane))
+        lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(r, lane), lm1[k + 1],
+                                            mu1[k + 1], taper(r, lane), dts, fx, fz,
+                                            n2[k][0], n2[k][1], n2[k][2]);
+      row_fence();
+    }
+    if (!fl.ok()) {
+#pragma unroll
+      for (int k = 0; k < K2; ++k) {
+        const int r = -2 + w + 4 * k, ib = L::B(r, lane);
+        n2[k][0] = s_xx[ib], n2[k][1] = s_zz[ib], n2[k][2] = s_xz[ib];
+        if (interior(r, lane))
+          lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(r, lane), lm1[k + 1],
+                                              mu1[k + 1], taper(r, lane), dts, ix, iz,
+                                              n2[k][0], n2[k][1], n2[k][2]);
+      }
+    }
+    double h[3] = {0.0, 0.0, 0.0};
+    if (has2) {
+      const int ib = L::B(s2r, s2c);
+      h[0] = s_xx[ib], h[1] = s_zz[ib], h[2] = s_xz[ib];
+      if (interior(s2r, s2c)) {
+        DivFlags sf;
+        const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &sf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &sf};
+        const double t = taper(s2r, s2c);
+        lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(s2r, s2c), s2lm, s2mu, t, dts,
+                                            gx, gz, h[0], h[1], h[2]);
+        if (!sf.ok()) {
+          h[0] = s_xx[ib], h[1] = s_zz[ib], h[2] = s_xz[ib];
+          lean_stress<kXX | kZZ | kXZ, L::AW>(s_vx, s_vz, L::A(s2r, s2c), s2lm, s2mu, t,
+                                              dts, ix, iz, h[0], h[1], h[2]);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K2; ++k) {
+      const int r = -2 + w + 4 * k, ib = L::B(r, lane);
+      s_xx[ib] = n2[k][0], s_zz[ib] = n2[k][1], s_xz[ib] = n2[k][2];
+      if (r >= 0 && r < TZ && xin && z0 + r < a.nz) {
+        const unsigned o = off(r, lane);
+        stb(txxo, o, n2[k][0]);
+        stb(tzzo, o, n2[k][1]);
+        stb(txzo, o, n2[k][2]);
+      }
+    }
+    if (has2) {
+      const int ib = L::B(s2r, s2c);
+      s_xx[ib] = h[0], s_zz[ib] = h[1], s_xz[ib] = h[2];
+    }
+  }
+  __syncthreads();
+
+  // ---- stage 4: v2 on the tile, step s1's injection, store ----
+  double vx[R], vz[R];
+  {
+    DivFlags fl;
+    const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int r = w + 4 * j, ia = L::A(r, lane);
+      vx[j] = s_vx[ia], vz[j] = s_vz[ia];
+      if (interior(r, lane))
+        l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(r, lane), bv[j + 1], taper(r, lane), dts,
+                           fx, fz, vx[j], vz[j]);
+      row_fence();
+    }
+    if (!fl.ok()) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j, ia = L::A(r, lane);
+        vx[j] = s_vx[ia], vz[j] = s_vz[ia];
+        if (interior(r, lane))
+          l2_velocity<L::BW>(s_xx, s_zz, s_xz, L::B(r, lane), bv[j + 1], taper(r, lane),
+                             dts, ix, iz, vx[j], vz[j]);
+      }
+    }
+  }
+  for (int e = 0; e < ne; ++e) {
+    int lc;
+    double coef, val;
+    if (e < kInjStage) {
+      lc = s_iloc[1][e], coef = s_icoef[1][e], val = s_ival[1][e];
+    } else {
+      const InjEntry E = a.entries[e0 + e];
+      lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E, a.it1);
+    }
+    const int local = lc & 0xffff, comp = lc >> 16;
+    const int r = local / kTileX, c = local - r * kTileX;
+    if ((r & 3) != w || c != lane) continue;
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (j == (r >> 2)) {
+        const double term = !a.adjoint
+                                ? coef * val * a.dt * bv[j + 1] / (comp == 0 ? a.d4x : a.d4z)
+                                : a.dt * bv[j + 1] * val / a.dxdz;
+        if (comp == 0)
+          vx[j] += term;
+        else
+          vz[j] += term;
+      }
+  }
+  double *vxo = a.vx_o + tb, *vzo = a.vz_o + tb;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r = w + 4 * j;
+    if (xin && z0 + r < a.nz) {
+      const unsigned o = off(r, lane);
+      stb(vxo, o, vx[j]);
+      stb(vzo, o, vz[j]);
+    }
+  }
+}
+
+template <int TZ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+step_lean2(StepArgs a) {
+  const int tile = block_tile(a);
+  if (tile >= a.ntiles) return;
+  lean2_tile<TZ>(a, tile);
+}
+
 // Two time steps per launch (s0 = a.it, s1 = a.it1).  The tile stages
 // velocities with an 8-point halo and stresses with a 6-point ring, then
 // updates in place in LDS: stresses of step s0 on the 6-ring, velocities on
@@ -1489,12 +1969,16 @@ void fast_division_constants(double d, double *zh, double *zl) {
   *zl = std::fma(-y, d, 1.0) / d;  // RN of the exact (1 - zh d) / d, >= 0
 }
 bool step_is_two_step(int variant) {
-  return variant == kTwoStepTz8 || variant == kTwoStepTz16;
+  return variant == kTwoStepTz8 || variant == kTwoStepTz16 || variant == kLeanTwoTz8;
 }
 int step_single_variant(int variant) {
   if (variant == kTwoStepTz8) return kFusedTz8;
   if (variant == kTwoStepTz16) return kFusedTz16;
+  if (variant == kLeanTwoTz8) return kLeanTz8;
   return variant;
+}
+bool step_uses_fastdiv(int variant) {
+  return step_is_lean(variant) || variant == kLeanTwoTz8;
 }
 
 hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
@@ -1505,6 +1989,8 @@ hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
     hipExtLaunchKernelGGL((step_two<8, 256>), g, dim3(256), 0, s, e0, e1, 0, a);
   else if (variant == kTwoStepTz16)
     hipExtLaunchKernelGGL((step_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, a);
+  else if (variant == kLeanTwoTz8)
+    hipExtLaunchKernelGGL((step_lean2<8>), g, dim3(256), 0, s, e0, e1, 0, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

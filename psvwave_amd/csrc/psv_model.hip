@@ -348,8 +348,8 @@ bool Model::lean_eligible() const {
 void Model::set_step_variant(int variant, int xcd_remap) {
   if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
-  if (step_is_lean(variant) && !lean_ok_)
-    throw ValueError("step variants 8-13 need dx and dz in [2^-40, 2^40] "
+  if (step_uses_fastdiv(variant) && !lean_ok_)
+    throw ValueError("step variants 8-14 need dx and dz in [2^-40, 2^40] "
                      "and a taper that grows towards the interior");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;

@@ -23,7 +23,8 @@ def _run(model, fields, obs):
 
 
 @pytest.mark.parametrize("variant,every,nt", [(0, 4, 300), (6, 8, 295), (2, 3, 300),
-                                              (7, 5, 300), (8, 4, 300)])
+                                              (7, 5, 300), (8, 4, 300),
+                                              (14, 5, 295)])
 def test_checkpointed_adjoint_is_bitwise_full_storage(oracle_lib, variant, every, nt):
     import psvwave_amd
     cfg = small_config(**{"domain.nt": nt})

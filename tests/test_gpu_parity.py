@@ -114,7 +114,7 @@ def _small_models(cfg):
     return true, start
 
 
-@pytest.mark.parametrize("variant", list(range(14)))
+@pytest.mark.parametrize("variant", list(range(15)))
 def test_small_chain_matches_oracle(oracle_lib, variant):
     cfg = small_config()
     true, start = _small_models(cfg)
@@ -124,7 +124,7 @@ def test_small_chain_matches_oracle(oracle_lib, variant):
     _compare(p, o)
 
 
-@pytest.mark.parametrize("variant", [6, 7])
+@pytest.mark.parametrize("variant", [6, 7, 14])
 def test_two_step_odd_nt_matches_oracle(oracle_lib, variant):
     """Two-step launches with an odd number of steps (the last step runs on
     the one-step kernel) and snapshot steps at odd time indices."""
