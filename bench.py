@@ -83,9 +83,13 @@ def _args():
                    help="first run a plain copy kernel over BYTES (the "
                         "reference point for byte counters) and report it")
     p.add_argument("--only-calibrate", action="store_true")
-    p.add_argument("--workload", choices=["single", "mc"], default="single",
+    p.add_argument("--workload", choices=["single", "mc", "c4"], default="single",
                    help="single: the layered-faulted model (default); mc: the "
-                        "512-model Monte-Carlo forward batch")
+                        "512-model Monte-Carlo forward batch; c4: gradient "
+                        "through torch.autograd with checkpointing vs full "
+                        "snapshot storage, checked by finite differences")
+    p.add_argument("--c4-directions", type=int, default=20)
+    p.add_argument("--c4-every", type=int, default=8)
     p.add_argument("--mc-models", type=int, default=512)
     return p.parse_args()
 
@@ -206,6 +210,94 @@ def monte_carlo(a):
     }
 
 
+def gradient_check(a):
+    """C4: the FWI gradient of a 680 x 380 Gaussian-random-field model (4
+    shots, 64 receivers, nt = 4000) through psvwave_amd.autograd, once with
+    every snapshot stored (16.5 GB) and once with checkpointing (every k-th
+    snapshot kept, the rest recomputed); the two must agree bit for bit.
+    The gradient is then compared with central differences of the misfit
+    along smooth random directions supported on the kernel region."""
+    import torch
+    import psvwave_amd
+    from psvwave_amd.autograd import misfit
+    from psvwave_amd.synthetic import c4_config, c4_models, gaussian_random_field
+    cfg = c4_config()
+    true, start = c4_models(cfg)
+    res = {}
+    grads = {}
+    obs = None
+    for every in (1, a.c4_every):
+        m = psvwave_amd.fdModel(cfg)
+        m.set_checkpointing(every)
+        if obs is None:
+            m.set_parameter_fields(*true)
+            for s in range(m.n_shots):
+                m.forward_simulate(s, store_fields=False)
+            obs = m.get_synthetic_data()
+        m.set_observed_data(*obs)
+        params = [torch.tensor(f, dtype=torch.float64, requires_grad=True) for f in start]
+        torch.cuda.synchronize() if torch.cuda.is_available() else None
+        t0 = time.perf_counter()
+        f = misfit(*params, m)
+        f.backward()
+        secs = time.perf_counter() - t0
+        grads[every] = [p.grad.numpy().copy() for p in params]
+        res[every] = {"seconds": round(secs, 3), "device_bytes": m.device_bytes(),
+                      "misfit": float(f.detach())}
+        _progress(f"gradient with checkpoint interval {every}: {secs:.2f} s")
+        if every == a.c4_every:
+            model = m
+    bitwise = all(np.array_equal(x, y) for x, y in zip(grads[1], grads[a.c4_every]))
+    # central differences along smooth random directions (each parameter
+    # perturbed by a smooth field of up to 1 % of its mean)
+    rng = np.random.default_rng(5)
+    shape = start[0].shape
+
+    def plain_misfit(fields):
+        model.set_parameter_fields(*fields)
+        for s in range(model.n_shots):
+            model.forward_simulate(s, store_fields=False)
+        model.calculate_l2_misfit()
+        return model.misfit
+
+    # the reference gradient is accumulated on the kernel region only
+    # (src/fdModel.cpp:817-861), so directions live there
+    dd_ = cfg["domain"]
+    npb = cfg["boundary"]["np_boundary"]
+    mask = np.zeros(shape)
+    mask[npb + dd_["nx_inner_boundary"]:npb + dd_["nx_inner"] - dd_["nx_inner_boundary"],
+         npb + dd_["nz_inner_boundary"]:npb + dd_["nz_inner"] - dd_["nz_inner_boundary"]] = 1.0
+    errors = []
+    for k in range(a.c4_directions):
+        d = [mask * gaussian_random_field(shape, 20, rng) * rng.choice([-1.0, 1.0]) * 0.01
+             * f.mean() for f in start]
+        h = 0.5
+        fp = plain_misfit([f + h * dd for f, dd in zip(start, d)])
+        fm = plain_misfit([f - h * dd for f, dd in zip(start, d)])
+        fd = (fp - fm) / (2 * h)
+        g = float(sum(np.sum(gg * dd) for gg, dd in zip(grads[a.c4_every], d)))
+        errors.append(abs(g - fd) / abs(fd))
+        _progress(f"direction {k + 1}/{a.c4_directions}: gradient {g:.6e} "
+                  f"finite difference {fd:.6e}")
+    full, ck = res[1], res[a.c4_every]
+    n = shape[0] * shape[1] * cfg["domain"]["nt"] * cfg["sources"]["n_shots"]
+    return {
+        "metric": "C4 FWI gradient through torch.autograd with checkpointing",
+        "value": round(3.0 * n / ck["seconds"] / 1e9, 3),
+        "unit": "Gpt-updates/s (forward + recompute + adjoint of the checkpointed gradient)",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded GRF)",
+        "config": {"workload": "C4 gradient check", "grid": f"{shape[0]}x{shape[1]}",
+                   "nt": cfg["domain"]["nt"], "shots": cfg["sources"]["n_shots"],
+                   "receivers": cfg["receivers"]["nr"], "checkpoint_interval": a.c4_every},
+        "full_storage": full, "checkpointed": ck,
+        "device_bytes_ratio": round(ck["device_bytes"] / full["device_bytes"], 4),
+        "bitwise_equal_gradients": bitwise,
+        "fd_relative_errors": [round(e, 5) for e in errors],
+        "fd_max_relative_error": round(max(errors), 5) if errors else None,
+        "fd_median_relative_error": round(float(np.median(errors)), 5) if errors else None,
+    }
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -223,6 +315,9 @@ def main():
             return
     if a.workload == "mc":
         print(json.dumps(monte_carlo(a)), flush=True)
+        return
+    if a.workload == "c4":
+        print(json.dumps(gradient_check(a)), flush=True)
         return
     import psvwave_amd
     from psvwave_amd.synthetic import (algorithmic_bytes_per_shot,

@@ -139,3 +139,46 @@ def monte_carlo_model(cfg, rng, corr_len=20):
     vs = vp * (0.45 + 0.10 * gaussian_random_field(shape, corr_len, rng))
     rho = 1400.0 + 800.0 * gaussian_random_field(shape, corr_len, rng)
     return vp, vs, rho
+
+
+def c4_config(nt=4000, n_shots=4, nr=64):
+    """Configuration of the gradient-check workload: 600 x 300 inner grid
+    plus a 40-point taper (680 x 380), dx = dz = 5 m, dt = 0.5 ms, 10 Hz,
+    n_shots shots of one source each spread along the top at depth 10, nr
+    receivers at depth 5, snapshot every 10 steps (with nt = 4000 and four
+    shots the full snapshot store is 16.5 GB)."""
+    nxi, nzi, npb = 600, 300, 40
+    return {
+        "domain": dict(nt=nt, nx_inner=nxi, nz_inner=nzi, nx_inner_boundary=10,
+                       nz_inner_boundary=10, dx=5.0, dz=5.0, dt=5e-4),
+        "boundary": dict(np_boundary=npb, np_factor=0.015),
+        "medium": dict(scalar_rho=1800.0, scalar_vp=2400.0, scalar_vs=1200.0),
+        "sources": dict(peak_frequency=10.0, n_sources=n_shots, n_shots=n_shots,
+                        source_timeshift=0.0, delay_cycles_per_shot=0,
+                        moment_angles=[45.0 + 30.0 * k for k in range(n_shots)],
+                        ix_sources=[int((k + 0.5) * nxi / n_shots) for k in range(n_shots)],
+                        iz_sources=[10] * n_shots,
+                        which_source_to_fire_in_which_shot=[[k] for k in range(n_shots)]),
+        "receivers": dict(nr=nr, ix_receivers=[int((k + 0.5) * nxi / nr) for k in range(nr)],
+                          iz_receivers=[5] * nr),
+        "inversion": dict(snapshot_interval=10),
+        "basis": dict(npx=1, npz=1),
+        "output": dict(observed_data_folder=".", stf_folder="."),
+    }
+
+
+def c4_models(cfg, seed=4):
+    """(true, start) models of the gradient-check workload: the start model
+    is a smooth Gaussian random field (vp 2200-2800 m/s, vs/vp 0.45-0.55,
+    rho 1600-2000 kg/m3, correlation length 40 points); the true model adds
+    a +-5 % perturbation with correlation length 12 points."""
+    rng = np.random.default_rng(seed)
+    d, npb = cfg["domain"], cfg["boundary"]["np_boundary"]
+    shape = (d["nx_inner"] + 2 * npb, d["nz_inner"] + 2 * npb)
+    vp = 2200.0 + 600.0 * gaussian_random_field(shape, 40, rng)
+    vs = vp * (0.45 + 0.10 * gaussian_random_field(shape, 40, rng))
+    rho = 1600.0 + 400.0 * gaussian_random_field(shape, 40, rng)
+    start = (vp, vs, rho)
+    true = tuple(f * (1.0 + 0.1 * (gaussian_random_field(shape, 12, rng) - 0.5))
+                 for f in start)
+    return true, start
