@@ -83,12 +83,15 @@ def _args():
                    help="first run a plain copy kernel over BYTES (the "
                         "reference point for byte counters) and report it")
     p.add_argument("--only-calibrate", action="store_true")
-    p.add_argument("--workload", choices=["single", "mc", "c4"], default="single",
+    p.add_argument("--workload", choices=["single", "mc", "c4", "c5"], default="single",
                    help="single: the layered-faulted model (default); mc: the "
                         "512-model Monte-Carlo forward batch; c4: gradient "
                         "through torch.autograd with checkpointing vs full "
                         "snapshot storage, checked by finite differences")
     p.add_argument("--c4-directions", type=int, default=20)
+    p.add_argument("--c5-pairs", type=int, default=2048)
+    p.add_argument("--c5-epochs", type=int, default=20)
+    p.add_argument("--c5-iterations", type=int, default=30)
     p.add_argument("--c4-every", type=int, default=8)
     p.add_argument("--mc-models", type=int, default=512)
     return p.parse_args()
@@ -298,6 +301,61 @@ def gradient_check(a):
     }
 
 
+def learned_fwi(a):
+    """C5: FWI time-to-misfit with the learned preconditioner against plain
+    steepest descent on the same GPU.  Training pairs (gradient at a smooth
+    background, true perturbation) come from the device solver; the U-Net is
+    trained in bfloat16 autocast; both FWI runs start from the background of
+    a held-out model and use the same line search.  `value` is the learned
+    run's time to reach the misfit the plain run ends at, divided by the
+    plain run's time (lower is better)."""
+    os.environ.setdefault("MIOPEN_FIND_MODE", "FAST")  # no exhaustive conv search
+    import torch
+    import psvwave_amd
+    from psvwave_amd import precond
+    t0 = time.perf_counter()
+    grads, perts = precond.gradient_pairs(a.c5_pairs, seed=7, workers=8, log=_progress)
+    t_pairs = time.perf_counter() - t0
+    _progress(f"{a.c5_pairs} training pairs in {t_pairs:.1f} s")
+    torch.manual_seed(0)
+    net = precond.make_unet()
+    t0 = time.perf_counter()
+    hist = precond.train(net, grads, perts, epochs=a.c5_epochs, log=_progress)
+    t_train = time.perf_counter() - t0
+    net.eval()
+    bg, true = precond.c5_models(np.random.default_rng(12345))
+    model = psvwave_amd.fdModel(precond.c5_config())
+    model.set_parameter_fields(*true)
+    for s in range(model.n_shots):
+        model.forward_simulate(s, store_fields=False)
+    model.set_observed_data(*model.get_synthetic_data())
+    h_plain, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
+                             log=lambda m: _progress("plain " + m))
+    target = h_plain[-1][1]
+    h_learn, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
+                             direction=lambda g, f: precond.learned_direction(net, g, f),
+                             target=target, log=lambda m: _progress("learned " + m))
+    t_plain = precond.time_to_misfit(h_plain, target)
+    t_learn = precond.time_to_misfit(h_learn, target)
+    return {
+        "metric": "FWI time-to-misfit, learned vs plain",
+        "value": round(t_learn / t_plain, 4) if t_learn is not None else None,
+        "unit": "learned time / plain time to the plain run's final misfit",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "f64 solver, bf16 network",
+        "data": "synthetic (seeded GRF background + perturbation)",
+        "config": {"workload": "C5 learned preconditioner", "grid": "316x188", "nt": 2500,
+                   "shots": 6, "receivers": 48, "training_pairs": a.c5_pairs,
+                   "epochs": a.c5_epochs, "fwi_iterations": a.c5_iterations},
+        "pairs_seconds": round(t_pairs, 1), "train_seconds": round(t_train, 1),
+        "train_loss": [round(x, 5) for x in hist],
+        "plain": {"seconds": round(h_plain[-1][0], 2), "initial_misfit": h_plain[0][1],
+                  "final_misfit": h_plain[-1][1], "iterations": len(h_plain) - 1},
+        "learned": {"seconds_to_target": round(t_learn, 2) if t_learn is not None else None,
+                    "final_misfit": h_learn[-1][1], "iterations": len(h_learn) - 1,
+                    "fallbacks": int(sum(1 for h in h_learn if h[2]))},
+    }
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -318,6 +376,9 @@ def main():
         return
     if a.workload == "c4":
         print(json.dumps(gradient_check(a)), flush=True)
+        return
+    if a.workload == "c5":
+        print(json.dumps(learned_fwi(a)), flush=True)
         return
     import psvwave_amd
     from psvwave_amd.synthetic import (algorithmic_bytes_per_shot,

@@ -1,0 +1,330 @@
+"""Learned gradient preconditioner and the FWI driver that compares it with
+plain steepest descent (time-to-misfit, learned vs plain).
+
+Workload: a 256 x 128 inner grid plus a 30-point taper (316 x 188),
+dx = dz = 10 m, dt = 1 ms, nt = 2500, 8 Hz, 6 shots and 48 receivers along
+the surface.  Models are a smooth Gaussian-random-field background plus a
++-5 % random perturbation inside the kernel region.
+
+* ``gradient_pairs``: (gradient at the background, true perturbation) pairs,
+  computed by several device models at once (one host thread and one HIP
+  stream each; the C-ABI releases the GIL).
+* ``UNet``: a 4-level U-Net (32 base channels) mapping the normalised
+  velocity kernels to a normalised model update; trained in bfloat16
+  autocast with fp32 master weights.
+* ``fwi``: steepest-descent FWI with a backtracking line search on the
+  plain-forward misfit; with a preconditioner, its update direction is
+  tried first and the plain gradient is the fallback when the line search
+  finds no decrease along it.
+
+The gradient is the reference's (``get_kernels`` after the adjoint,
+src/fdModel.cpp:789-969, 1164-1183); the FWI loop follows the descent loop of
+tests/test_descent.py:106-129 with a line search added.
+"""
+import threading
+import time
+
+import numpy as np
+
+from .synthetic import gaussian_random_field
+
+NP_BOUNDARY = 30
+NX_INNER, NZ_INNER = 256, 128
+NZ_INNER_BOUNDARY = 8  # sources and receivers sit above the kernel region
+
+
+def c5_config(nt=2500, n_shots=6, nr=48):
+    """Configuration dict of the preconditioner workload (316 x 188 grid)."""
+    nxi, nzi = NX_INNER, NZ_INNER
+    return {
+        "domain": dict(nt=nt, nx_inner=nxi, nz_inner=nzi, nx_inner_boundary=0,
+                       nz_inner_boundary=NZ_INNER_BOUNDARY, dx=10.0, dz=10.0, dt=1e-3),
+        "boundary": dict(np_boundary=NP_BOUNDARY, np_factor=0.015),
+        "medium": dict(scalar_rho=1900.0, scalar_vp=2300.0, scalar_vs=1150.0),
+        "sources": dict(peak_frequency=8.0, n_sources=n_shots, n_shots=n_shots,
+                        source_timeshift=0.0, delay_cycles_per_shot=0,
+                        moment_angles=[30.0 + 20.0 * k for k in range(n_shots)],
+                        ix_sources=[int((k + 0.5) * nxi / n_shots) for k in range(n_shots)],
+                        iz_sources=[4] * n_shots,
+                        which_source_to_fire_in_which_shot=[[k] for k in range(n_shots)]),
+        "receivers": dict(nr=nr, ix_receivers=[int((k + 0.5) * nxi / nr) for k in range(nr)],
+                          iz_receivers=[2] * nr),
+        "inversion": dict(snapshot_interval=10),
+        "basis": dict(npx=1, npz=1),
+        "output": dict(observed_data_folder=".", stf_folder="."),
+    }
+
+
+def kernel_mask(shape):
+    """1 on the kernel region (where the reference accumulates kernels)."""
+    m = np.zeros(shape)
+    m[NP_BOUNDARY:NP_BOUNDARY + NX_INNER,
+      NP_BOUNDARY + NZ_INNER_BOUNDARY:NP_BOUNDARY + NZ_INNER - NZ_INNER_BOUNDARY] = 1.0
+    return m
+
+
+def c5_models(rng):
+    """(background, true) models: smooth background (vp 2000-2600 m/s,
+    vs/vp 0.48-0.52, rho 1800-2050 kg/m3, correlation length 40 points)
+    and the true model = background * (1 + perturbation), the perturbation
+    a +-5 % field (correlation length 8-16 points) on the kernel region."""
+    shape = (NX_INNER + 2 * NP_BOUNDARY, NZ_INNER + 2 * NP_BOUNDARY)
+    vp = 2000.0 + 600.0 * gaussian_random_field(shape, 40, rng)
+    vs = vp * (0.48 + 0.04 * gaussian_random_field(shape, 40, rng))
+    rho = 1800.0 + 250.0 * gaussian_random_field(shape, 40, rng)
+    mask = kernel_mask(shape)
+    corr = float(rng.uniform(8, 16))
+    p = 0.1 * (gaussian_random_field(shape, corr, rng) - 0.5) * mask
+    q = 0.1 * (gaussian_random_field(shape, corr, rng) - 0.5) * mask
+    true = (vp * (1 + p), vs * (1 + q), rho * (1 + 0.5 * p))
+    return (vp, vs, rho), true
+
+
+def _simulate(model, fields, store):
+    model.set_parameter_fields(*fields)
+    for s in range(model.n_shots):
+        model.forward_simulate(s, store_fields=store)
+
+
+def gradient(model):
+    """Velocity kernels at the model's current parameters against its
+    observed data (the misfit is left in model.misfit)."""
+    for s in range(model.n_shots):
+        model.forward_simulate(s, store_fields=True)
+    model.calculate_l2_misfit()
+    model.calculate_l2_adjoint_sources()
+    model.reset_kernels()
+    for s in range(model.n_shots):
+        model.adjoint_simulate(s)
+    model.map_kernels_to_velocity()
+    return model.get_kernels()
+
+
+def misfit_at(model, fields):
+    _simulate(model, fields, store=False)
+    model.calculate_l2_misfit()
+    return model.misfit
+
+
+def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None):
+    """n (gradient, perturbation) pairs as float32 arrays of shape
+    (n, 3, nx, nz): the velocity kernels at the background model against
+    data of the true model, and (true - background) / background."""
+    from .model import fdModel
+    cfg = cfg or c5_config()
+    rngs = [np.random.default_rng([seed, k]) for k in range(n)]
+    out_g = [None] * n
+    out_p = [None] * n
+    next_k = [0]
+    lock = threading.Lock()
+    errors = []
+
+    def work():
+        try:
+            m = fdModel(cfg)
+            while True:
+                with lock:
+                    k = next_k[0]
+                    next_k[0] += 1
+                    if log and k < n and k % 128 == 0:
+                        log(f"training pair {k}/{n}")
+                if k >= n:
+                    return
+                bg, true = c5_models(rngs[k])
+                _simulate(m, true, store=False)
+                m.set_observed_data(*m.get_synthetic_data())
+                m.set_parameter_fields(*bg)
+                g = gradient(m)
+                out_g[k] = np.stack(g).astype(np.float32)
+                out_p[k] = np.stack([(t - b) / b for t, b in zip(true, bg)]).astype(np.float32)
+        except Exception as e:  # surfaced in the caller
+            errors.append(e)
+
+    threads = [threading.Thread(target=work) for _ in range(max(1, workers))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return np.stack(out_g), np.stack(out_p)
+
+
+def normalise_gradient(g):
+    """Per-sample, per-parameter scaling of velocity kernels to unit RMS on
+    the kernel region (the network sees shapes, not magnitudes)."""
+    g = np.asarray(g, dtype=np.float32)
+    single = g.ndim == 3
+    if single:
+        g = g[None]
+    rms = np.sqrt(np.mean(g.astype(np.float64) ** 2, axis=(2, 3), keepdims=True))
+    out = (g / np.where(rms > 0, rms, 1.0)).astype(np.float32)
+    return out[0] if single else out
+
+
+def _unet_class():
+    import torch
+    from torch import nn
+
+    def block(ci, co):
+        return nn.Sequential(nn.Conv2d(ci, co, 3, padding=1), nn.GELU(),
+                             nn.Conv2d(co, co, 3, padding=1), nn.GELU())
+
+    class UNet(nn.Module):
+        """4-level U-Net, 32 base channels; input and output 3 channels
+        (vp, vs, rho) on the full (nx, nz) grid, zero-padded to a multiple
+        of 16."""
+
+        def __init__(self, base=32):
+            super().__init__()
+            c = [base, 2 * base, 4 * base, 8 * base]
+            self.enc = nn.ModuleList([block(3, c[0]), block(c[0], c[1]), block(c[1], c[2])])
+            self.mid = block(c[2], c[3])
+            self.up = nn.ModuleList([nn.ConvTranspose2d(c[3], c[2], 2, stride=2),
+                                     nn.ConvTranspose2d(c[2], c[1], 2, stride=2),
+                                     nn.ConvTranspose2d(c[1], c[0], 2, stride=2)])
+            self.dec = nn.ModuleList([block(2 * c[2], c[2]), block(2 * c[1], c[1]),
+                                      block(2 * c[0], c[0])])
+            self.out = nn.Conv2d(c[0], 3, 1)
+
+        def forward(self, x):
+            nx, nz = x.shape[-2:]
+            px, pz = (-nx) % 16, (-nz) % 16
+            x = nn.functional.pad(x, (0, pz, 0, px))
+            skips = []
+            for e in self.enc:
+                x = e(x)
+                skips.append(x)
+                x = nn.functional.max_pool2d(x, 2)
+            x = self.mid(x)
+            for u, d, s in zip(self.up, self.dec, reversed(skips)):
+                x = d(torch.cat([u(x), s], dim=1))
+            return self.out(x)[..., :nx, :nz]
+
+    return UNet
+
+
+def make_unet(base=32):
+    return _unet_class()(base)
+
+
+def train(net, grads, perts, epochs=20, batch=16, lr=2e-3, device="cuda", log=None):
+    """Fit the network to map normalised gradients to perturbations
+    (mean-squared error on the kernel region), bfloat16 autocast."""
+    import torch
+    net = net.to(device)
+    mask = torch.tensor(kernel_mask(grads.shape[-2:]), dtype=torch.float32, device=device)
+    x_all = torch.tensor(normalise_gradient(grads), device=device)
+    y_all = torch.tensor(perts, device=device)
+    opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=1e-4)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=epochs)
+    n = x_all.shape[0]
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    history = []
+    for ep in range(epochs):
+        te = time.perf_counter()
+        perm = torch.randperm(n, generator=gen)
+        tot = 0.0
+        for i in range(0, n, batch):
+            idx = perm[i:i + batch].to(device)
+            x, y = x_all[idx], y_all[idx]
+            if torch.rand(1, generator=gen).item() < 0.5:  # mirror in x
+                x, y = torch.flip(x, dims=[2]), torch.flip(y, dims=[2])
+            with torch.autocast(device_type="cuda", dtype=torch.bfloat16,
+                                enabled=device.startswith("cuda")):
+                pred = net(x)
+            loss = (((pred.float() - y) * mask) ** 2).mean() / (mask.mean() * 1e-3)
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            opt.step()
+            tot += float(loss.detach()) * x.shape[0]
+        sched.step()
+        history.append(tot / n)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {history[-1]:.5f} "
+                f"({time.perf_counter() - te:.1f} s)")
+    return history
+
+
+def learned_direction(net, g, fields, device="cuda"):
+    """Model update direction from the network: the predicted relative
+    perturbation times the current parameters (descent direction up to the
+    line search's scale)."""
+    import torch
+    x = torch.tensor(normalise_gradient(np.stack(g))[None], device=device)
+    with torch.no_grad(), torch.autocast(device_type="cuda", dtype=torch.bfloat16,
+                                         enabled=device.startswith("cuda")):
+        y = net(x)
+    y = y.float()[0].cpu().numpy().astype(np.float64)
+    return [yy * f for yy, f in zip(y, fields)]
+
+
+def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
+        target=None, log=None):
+    """Steepest-descent FWI from `start` against model's observed data.
+
+    Each iteration computes the gradient and tries, in order, the
+    preconditioned update `direction(g, fields)` (when given) at full scale
+    and the plain direction (-g, each parameter block scaled so a step
+    changes it by at most step0 times its largest value, times an adaptive
+    step factor), halving each until the plain-forward misfit decreases (the
+    plain direction is the fallback).  Returns the history: list of
+    (elapsed seconds, misfit, used fallback)."""
+    fields = [np.array(f, dtype=np.float64) for f in start]
+    model.set_parameter_fields(*fields)
+    t0 = time.perf_counter()
+    hist = []
+    factor = 1.0
+    mask = kernel_mask(fields[0].shape)
+    for it in range(iterations):
+        g = gradient(model)
+        f0 = model.misfit
+        if it == 0:
+            hist.append((time.perf_counter() - t0, f0, False))
+        cands = []
+        if direction is not None:
+            cands.append(("learned", direction(g, fields)))
+        # plain direction: -g with each parameter block scaled to its own
+        # largest value, so one step changes vp, vs and rho by the same
+        # fraction (raw kernels differ by orders of magnitude between them)
+        cands.append(("plain", [-gg / max(np.abs(gg).max(), 1e-300) * f.max()
+                                for gg, f in zip(g, fields)]))
+        accepted = False
+        for name, p in cands:
+            p = [pp * mask for pp in p]
+            if name == "learned":
+                # the network predicts the update itself: try it at full scale
+                scale, a = 1.0, 1.0
+            else:
+                scale = step0
+                a = scale * factor
+            for _ in range(max_halvings):
+                trial = [f + a * pp for f, pp in zip(fields, p)]
+                ft = misfit_at(model, trial)
+                if ft < f0:
+                    fields = trial
+                    if name == "plain":
+                        factor = min(2.0 * a / scale, 4.0)
+                    accepted = True
+                    break
+                a *= 0.5
+            if accepted:
+                hist.append((time.perf_counter() - t0, ft, name == "plain" and direction is not None))
+                break
+        if not accepted:
+            hist.append((time.perf_counter() - t0, f0, True))
+            factor *= 0.25
+        model.set_parameter_fields(*fields)
+        if log:
+            log(f"iteration {it + 1}: misfit {hist[-1][1]:.6e} ({hist[-1][0]:.2f} s)")
+        if target is not None and hist[-1][1] <= target:
+            break
+    return hist, fields
+
+
+def time_to_misfit(hist, target):
+    """Seconds until the misfit first reaches `target` (None if never)."""
+    for t, f, _ in hist:
+        if f <= target:
+            return t
+    return None
