@@ -264,8 +264,9 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
     """Steepest-descent FWI from `start` against model's observed data.
 
     Each iteration computes the gradient and tries, in order, the
-    preconditioned update `direction(g, fields)` (when given) at full scale
-    and the plain direction (-g, each parameter block scaled so a step
+    preconditioned update `direction(g, fields)` (when given; first at its
+    own scale, later at 1.5 x the last accepted fraction of it) and the
+    plain direction (-g, each parameter block scaled so a step
     changes it by at most step0 times its largest value, times an adaptive
     step factor), halving each until the plain-forward misfit decreases (the
     plain direction is the fallback).  Returns the history: list of
@@ -274,7 +275,8 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
     model.set_parameter_fields(*fields)
     t0 = time.perf_counter()
     hist = []
-    factor = 1.0
+    factor = 1.0   # plain step factor (adaptive)
+    lfactor = 1.0  # learned step scale (adaptive, at most the network's own)
     mask = kernel_mask(fields[0].shape)
     for it in range(iterations):
         g = gradient(model)
@@ -293,8 +295,9 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
         for name, p in cands:
             p = [pp * mask for pp in p]
             if name == "learned":
-                # the network predicts the update itself: try it at full scale
-                scale, a = 1.0, 1.0
+                # the network predicts the update itself: start from its own
+                # scale, or from 1.5 x the last accepted fraction of it
+                scale, a = 1.0, lfactor
             else:
                 scale = step0
                 a = scale * factor
@@ -305,9 +308,13 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
                     fields = trial
                     if name == "plain":
                         factor = min(2.0 * a / scale, 4.0)
+                    else:
+                        lfactor = min(1.5 * a, 1.0)
                     accepted = True
                     break
                 a *= 0.5
+            if not accepted and name == "learned":
+                lfactor = max(a, 1.0 / 64)
             if accepted:
                 hist.append((time.perf_counter() - t0, ft, name == "plain" and direction is not None))
                 break
