@@ -255,6 +255,24 @@ int psv_batch_autotune(psv_batch *b, int steps, int *best,
 int psv_batch_timing(psv_batch *b, double *loop_ms, int64_t *points,
                      int reset);
 
+/* Learned gradient preconditioner (no reference counterpart; the reference
+ * has no preconditioner).  bfloat16 MFMA convolution on device buffers the
+ * caller owns, enqueued on `stream` (a hipStream_t; NULL = the default
+ * stream) and not synchronised.  Activations NHWC (one image) as bfloat16
+ * bit patterns; weights [N][K] with K = (ky * ks + kx) * cin + ci; cin a
+ * multiple of 8; ks 1 or 3 ("same" padding).  transposed = 1: 2x2
+ * stride-2 transposed convolution (ks = 1, N = 4 * cout, n = (2a + b) *
+ * cout + co writes pixel (2y + a, 2x + b) of a (2h, 2w) output).  The
+ * result (+ bias, then erf GELU if gelu = 1) goes to channel out_offset of
+ * rows of out_stride channels, fp32 if out_f32 = 1 else bfloat16. */
+int psv_conv2d_bf16(const void *in, const void *weights, const float *bias,
+                    void *out, int out_f32, int h, int w, int cin, int cout,
+                    int ks, int transposed, int gelu, int out_stride,
+                    int out_offset, void *stream);
+/* 2 x 2 max pooling of an NHWC bfloat16 image (h, w even). */
+int psv_maxpool2_bf16(const void *in, void *out, int h, int w, int c,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -126,6 +126,9 @@ _SIGNATURES = {
     "psv_batch_set_graph_mode": [_P, _i],
     "psv_batch_autotune": [_P, _i, C.POINTER(C.c_int), _D],
     "psv_batch_timing": [_P, _D, C.POINTER(C.c_int64), _i],
+    "psv_conv2d_bf16": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i, _i, _i,
+                        _i, _i, _i, _i, _i, _i, C.c_void_p],
+    "psv_maxpool2_bf16": [C.c_void_p, C.c_void_p, _i, _i, _i, C.c_void_p],
 }
 _RESTYPES = {"psv_last_error": C.c_char_p, "psv_config_destroy": None,
              "psv_destroy": None, "psv_batch_destroy": None}

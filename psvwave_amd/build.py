@@ -16,7 +16,7 @@ ARCH = os.environ.get("PSV_OFFLOAD_ARCH", "gfx950")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 SOURCES = ["psv_config.cpp", "psv_kernels.hip", "psv_model.hip",
-           "psv_batch.hip", "psv_capi.cpp"]
+           "psv_batch.hip", "psv_conv.hip", "psv_capi.cpp"]
 
 # -ffp-contract=off: every multiply and add rounds separately, as in the
 # reference's x86-64 build, so stencil results match it bit for bit.

@@ -725,3 +725,36 @@ int psv_batch_timing(psv_batch *b, double *loop_ms, int64_t *points,
 }
 
 }  // extern "C"
+
+int psv_conv2d_bf16(const void *in, const void *weights, const float *bias, void *out,
+                    int out_f32, int h, int w, int cin, int cout, int ks, int transposed,
+                    int gelu, int out_stride, int out_offset, void *stream) {
+  return guard([&] {
+    need(in, "in");
+    need(weights, "weights");
+    need(bias, "bias");
+    need(out, "out");
+    if (cin <= 0 || cin % 8 != 0) throw psv::ValueError("cin must be a positive multiple of 8");
+    if (ks != 1 && ks != 3) throw psv::ValueError("ks must be 1 or 3");
+    if (transposed && ks != 1) throw psv::ValueError("transposed convolutions use ks = 1");
+    if (h <= 0 || w <= 0 || cout <= 0) throw psv::ValueError("empty convolution");
+    if (out_offset < 0 || out_offset + cout > out_stride)
+      throw psv::ValueError("output channels exceed out_stride");
+    hipError_t e = psv::launch_conv_bf16(
+        (const uint16_t *)in, (const uint16_t *)weights, bias, out, out_f32 != 0, h, w, cin,
+        cout, ks, transposed != 0, gelu != 0, out_stride, out_offset, (hipStream_t)stream);
+    if (e != hipSuccess) throw psv::RuntimeError(std::string("conv2d: ") + hipGetErrorString(e));
+  });
+}
+
+int psv_maxpool2_bf16(const void *in, void *out, int h, int w, int c, void *stream) {
+  return guard([&] {
+    need(in, "in");
+    need(out, "out");
+    if (h <= 0 || w <= 0 || h % 2 || w % 2 || c <= 0)
+      throw psv::ValueError("maxpool2 needs positive even h, w and c > 0");
+    hipError_t e = psv::launch_maxpool2_bf16((const uint16_t *)in, (uint16_t *)out, h, w, c,
+                                             (hipStream_t)stream);
+    if (e != hipSuccess) throw psv::RuntimeError(std::string("maxpool2: ") + hipGetErrorString(e));
+  });
+}

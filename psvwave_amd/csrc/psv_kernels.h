@@ -231,6 +231,17 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
                              hipStream_t stream, hipEvent_t e0 = nullptr,
                              hipEvent_t e1 = nullptr);
 // Running sums of recorded receiver increments, in place: [nr][nt] x 2.
+// bfloat16 MFMA convolution for the preconditioner network (psv_conv.hip):
+// NHWC activations, weights [N][K], K = (ky * ks + kx) * cin + ci, cin a
+// multiple of 8; `transposed`: 2x2 stride-2 transposed convolution (ks = 1,
+// N = 4 cout).  Output bf16 or fp32 at channel offset out_off of rows of
+// out_stride channels; optional fused erf GELU.
+hipError_t launch_conv_bf16(const uint16_t *in, const uint16_t *w, const float *bias,
+                            void *out, bool out_f32, int h, int wd, int cin, int cout,
+                            int ks, bool transposed, bool gelu, int out_stride,
+                            int out_off, hipStream_t s);
+hipError_t launch_maxpool2_bf16(const uint16_t *in, uint16_t *out, int h, int w, int c,
+                                hipStream_t s);
 hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
                                 hipStream_t s);
 

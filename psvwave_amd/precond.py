@@ -335,3 +335,96 @@ def time_to_misfit(hist, target):
         if f <= target:
             return t
     return None
+
+
+class HipUNet:
+    """Inference of a trained ``UNet`` with this library's bfloat16 MFMA
+    kernels (``psv_conv2d_bf16``, ``psv_maxpool2_bf16``): NHWC bfloat16
+    activations, fp32 accumulation, bias and erf GELU fused; torch only
+    allocates the buffers and does the concatenations' copies."""
+
+    def __init__(self, net, device="cuda"):
+        import torch
+        from . import _lib
+        self._lib = _lib.load()
+        self._check = _lib.check
+        self.device = device
+        bf = torch.bfloat16
+
+        def conv_w(conv):
+            w = conv.weight.detach().float().cpu()  # (co, ci, k, k)
+            co, ci, kh, kw = w.shape
+            cp = -(-ci // 8) * 8
+            wp = torch.zeros(co, kh, kw, cp)
+            wp[..., :ci] = w.permute(0, 2, 3, 1)
+            return (wp.reshape(co, -1).to(device, bf).contiguous(),
+                    conv.bias.detach().float().to(device).contiguous(), cp, co, kh)
+
+        def convt_w(ct):
+            w = ct.weight.detach().float().cpu()  # (ci, co, 2, 2)
+            ci, co = w.shape[:2]
+            wp = w.permute(2, 3, 1, 0).reshape(4 * co, ci)  # n = (2a + b) co + o
+            return (wp.to(device, bf).contiguous(),
+                    ct.bias.detach().float().to(device).contiguous(), ci, co, 1)
+
+        self.enc = [(conv_w(b[0]), conv_w(b[2])) for b in net.enc]
+        self.mid = (conv_w(net.mid[0]), conv_w(net.mid[2]))
+        self.up = [convt_w(u) for u in net.up]
+        self.dec = [(conv_w(b[0]), conv_w(b[2])) for b in net.dec]
+        self.out = conv_w(net.out)
+
+    def _conv(self, x, layer, h, w, gelu=True, out=None, stride=None, off=0,
+              f32=False, transposed=False):
+        import torch
+        wt, b, cin, cout, k = layer
+        if out is None:
+            oh, ow = (2 * h, 2 * w) if transposed else (h, w)
+            out = torch.empty(oh, ow, cout, device=self.device,
+                              dtype=torch.float32 if f32 else torch.bfloat16)
+            stride = cout
+        self._check(self._lib.psv_conv2d_bf16(
+            x.data_ptr(), wt.data_ptr(), b.data_ptr(), out.data_ptr(), int(f32), h, w, cin,
+            cout, 1 if transposed else k, int(transposed), int(gelu), stride, off,
+            torch.cuda.current_stream().cuda_stream))
+        return out
+
+    def _pool(self, x, h, w, c):
+        import torch
+        out = torch.empty(h // 2, w // 2, c, device=self.device, dtype=torch.bfloat16)
+        self._check(self._lib.psv_maxpool2_bf16(x.data_ptr(), out.data_ptr(), h, w, c,
+                                                torch.cuda.current_stream().cuda_stream))
+        return out
+
+    def __call__(self, x):
+        """x: (3, nx, nz) array or tensor (normalised gradient); returns the
+        (3, nx, nz) float32 tensor the network predicts."""
+        import torch
+        x = torch.as_tensor(np.asarray(x, dtype=np.float32) if not torch.is_tensor(x) else x)
+        nx, nz = x.shape[-2:]
+        h, w = nx + (-nx) % 16, nz + (-nz) % 16
+        cp = self.enc[0][0][2]
+        a = torch.zeros(h, w, cp, device=self.device, dtype=torch.bfloat16)
+        a[:nx, :nz, :3] = x.to(self.device).permute(1, 2, 0).to(torch.bfloat16)
+        skips = []
+        for c1, c2 in self.enc:
+            a = self._conv(self._conv(a, c1, h, w), c2, h, w)
+            skips.append((a, h, w))
+            a = self._pool(a, h, w, c2[3])
+            h, w = h // 2, w // 2
+        a = self._conv(self._conv(a, self.mid[0], h, w), self.mid[1], h, w)
+        for up, (c1, c2), (s, sh, sw) in zip(self.up, self.dec, reversed(skips)):
+            cs = s.shape[-1]
+            cat = torch.empty(sh, sw, up[3] + cs, device=self.device, dtype=torch.bfloat16)
+            self._conv(a, up, h, w, gelu=False, out=cat, stride=up[3] + cs, off=0,
+                       transposed=True)
+            cat[:, :, up[3]:] = s
+            h, w = sh, sw
+            a = self._conv(self._conv(cat, c1, h, w), c2, h, w)
+        y = self._conv(a, self.out, h, w, gelu=False, f32=True)
+        return y[:nx, :nz].permute(2, 0, 1).contiguous()
+
+
+def learned_direction_hip(unet, g, fields):
+    """As learned_direction, with the network run by HipUNet."""
+    y = unet(normalise_gradient(np.stack(g))).cpu().numpy().astype(np.float64)
+    return [yy * f for yy, f in zip(y, fields)]
