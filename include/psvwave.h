@@ -269,9 +269,54 @@ int psv_conv2d_bf16(const void *in, const void *weights, const float *bias,
                     void *out, int out_f32, int h, int w, int cin, int cout,
                     int ks, int transposed, int gelu, int out_stride,
                     int out_offset, void *stream);
-/* 2 x 2 max pooling of an NHWC bfloat16 image (h, w even). */
+/* 2 x 2 max pooling of an NHWC bfloat16 image (h, w even; a batch of
+ * images stacked along the rows pools as one image of batch * h rows). */
 int psv_maxpool2_bf16(const void *in, void *out, int h, int w, int c,
                       void *stream);
+
+/* Training kernels of the same network (bfloat16 operands, fp32
+ * accumulation).  Images are stacked along the rows: `batch` images of
+ * h x w pixels.  General convolution: the input has in_stride channels per
+ * pixel, of which the first cin (a multiple of 8) are read.
+ *   mode 0: ks x ks convolution (ks 1 or 3, "same" padding per image),
+ *           K = ks * ks * cin, output h x w;
+ *   mode 1: 2x2 stride-2 transposed convolution of an h x w input
+ *           (ks = 1, N = 4 * cout), output 2h x 2w;
+ *   mode 2: 2x2 gather: the input has 2h x 2w pixels, K = 4 * cin and
+ *           k = (2a + b) * cin + ci reads pixel (2y + a, 2x + b) -- the
+ *           input gradient of mode 1 (weights [ci][(2a + b) * cout + co]).
+ * bias may be NULL.  pre_out (mode 0, may be NULL) receives the bfloat16
+ * pre-activation [pixels][cout]; gelu = 1 applies the erf GELU to it.  dz
+ * (may be NULL) multiplies the result by gelu'(dz[pixel * dz_stride + co])
+ * (the backward pass through a GELU whose pre-activation is dz). */
+int psv_conv_bf16(const void *in, int in_stride, const void *weights, const float *bias,
+                  void *out, int out_f32, int out_stride, int out_offset, void *pre_out,
+                  const void *dz, int dz_stride, int batch, int h, int w, int cin, int cout,
+                  int ks, int mode, int gelu, void *stream);
+/* Weight gradient of a mode-0 convolution, dw[co][(ky * ks + kx) *
+ * x_channels + ci] = sum over pixels of dy[p][co] x(p shifted by the tap)[ci]
+ * (dy_gather2x2 = 0, dy at the h x w pixels), or of a mode-1 transposed
+ * convolution, dw[(2a + b) * dy_channels + co][ci] (dy_gather2x2 = 1: dy at
+ * 2h x 2w pixels, ks = 1).  fp32 result; the pixel sum is split over
+ * workspace (up to workspace_floats floats; NULL = no split) and added in a
+ * fixed order, so the result is deterministic. */
+int psv_conv_wgrad_bf16(const void *dy, int dy_stride, int dy_gather2x2, int dy_channels,
+                        const void *x, int x_stride, int x_channels, int ks, int batch, int h,
+                        int w, float *dw, float *workspace, int64_t workspace_floats,
+                        void *stream);
+/* out[c] = sum over pixels of src[p * stride + offset + c] (bias gradients),
+ * fixed-order fp32 sums. */
+int psv_channel_sum_bf16(const void *src, int stride, int offset, int channels,
+                         int64_t pixels, float *out, float *workspace,
+                         int64_t workspace_floats, void *stream);
+/* Gradient at a block output feeding a 2x2 max pooling and a skip
+ * connection, through the block's GELU: out = (route(dpool) + dskip) *
+ * gelu'(z), route sending each pooled gradient (fp32, may be NULL) to the
+ * first maximum of its window of act in row-major order; dskip (bfloat16,
+ * may be NULL) at channel skip_offset of rows of skip_stride channels. */
+int psv_maxpool2_backward_bf16(const void *act, const void *z, const float *dpool,
+                               const void *dskip, int skip_stride, int skip_offset, void *out,
+                               int batch, int h, int w, int c, void *stream);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,14 @@ _SIGNATURES = {
     "psv_conv2d_bf16": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i, _i, _i,
                         _i, _i, _i, _i, _i, _i, C.c_void_p],
     "psv_maxpool2_bf16": [C.c_void_p, C.c_void_p, _i, _i, _i, C.c_void_p],
+    "psv_conv_bf16": [C.c_void_p, _i, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i, _i,
+                      C.c_void_p, C.c_void_p, _i, _i, _i, _i, _i, _i, _i, _i, _i, C.c_void_p],
+    "psv_conv_wgrad_bf16": [C.c_void_p, _i, _i, _i, C.c_void_p, _i, _i, _i, _i, _i, _i,
+                            C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+    "psv_channel_sum_bf16": [C.c_void_p, _i, _i, _i, C.c_int64, C.c_void_p, C.c_void_p,
+                             C.c_int64, C.c_void_p],
+    "psv_maxpool2_backward_bf16": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i,
+                                   C.c_void_p, _i, _i, _i, _i, C.c_void_p],
 }
 _RESTYPES = {"psv_last_error": C.c_char_p, "psv_config_destroy": None,
              "psv_destroy": None, "psv_batch_destroy": None}

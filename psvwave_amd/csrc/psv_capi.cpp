@@ -758,3 +758,122 @@ int psv_maxpool2_bf16(const void *in, void *out, int h, int w, int c, void *stre
     if (e != hipSuccess) throw psv::RuntimeError(std::string("maxpool2: ") + hipGetErrorString(e));
   });
 }
+
+int psv_conv_bf16(const void *in, int in_stride, const void *weights, const float *bias,
+                  void *out, int out_f32, int out_stride, int out_offset, void *pre_out,
+                  const void *dz, int dz_stride, int batch, int h, int w, int cin, int cout,
+                  int ks, int mode, int gelu, void *stream) {
+  return guard([&] {
+    need(in, "in");
+    need(weights, "weights");
+    need(out, "out");
+    if (mode < psv::kConv || mode > psv::kGather2x2)
+      throw psv::ValueError("mode must be 0 (convolution), 1 (transposed) or 2 (2x2 gather)");
+    if (cin <= 0 || cin % 8 != 0) throw psv::ValueError("cin must be a positive multiple of 8");
+    if (in_stride < cin || in_stride % 8 != 0)
+      throw psv::ValueError("in_stride must be a multiple of 8 and at least cin");
+    if (mode == psv::kConv ? (ks != 1 && ks != 3) : ks != 1)
+      throw psv::ValueError("ks must be 1 or 3 (1 for modes 1 and 2)");
+    if (batch <= 0 || h <= 0 || w <= 0 || cout <= 0) throw psv::ValueError("empty convolution");
+    if (out_offset < 0 || out_offset + cout > out_stride)
+      throw psv::ValueError("output channels exceed out_stride");
+    if (pre_out && mode != psv::kConv)
+      throw psv::ValueError("pre-activations are kept for mode 0 only");
+    if (dz && dz_stride < cout) throw psv::ValueError("dz_stride must be at least cout");
+    psv::ConvGemm a{};
+    a.in = (const uint16_t *)in;
+    a.in_stride = in_stride;
+    a.w = (const uint16_t *)weights;
+    a.bias = bias;
+    a.out16 = out_f32 ? nullptr : (uint16_t *)out;
+    a.out32 = out_f32 ? (float *)out : nullptr;
+    a.out_stride = out_stride;
+    a.out_off = out_offset;
+    a.pre16 = (uint16_t *)pre_out;
+    a.dz = (const uint16_t *)dz;
+    a.dz_stride = dz_stride;
+    a.rows = batch * h;
+    a.H = h;
+    a.W = w;
+    a.cin = cin;
+    a.cout = cout;
+    a.ks = ks;
+    a.mode = mode;
+    a.gelu = gelu ? 1 : 0;
+    hipError_t e = psv::launch_conv_gemm(a, (hipStream_t)stream);
+    if (e != hipSuccess) throw psv::RuntimeError(std::string("conv: ") + hipGetErrorString(e));
+  });
+}
+
+int psv_conv_wgrad_bf16(const void *dy, int dy_stride, int dy_gather2x2, int dy_channels,
+                        const void *x, int x_stride, int x_channels, int ks, int batch, int h,
+                        int w, float *dw, float *workspace, int64_t workspace_floats,
+                        void *stream) {
+  return guard([&] {
+    need(dy, "dy");
+    need(x, "x");
+    need(dw, "dw");
+    if (dy_channels <= 0 || dy_stride < dy_channels)
+      throw psv::ValueError("dy_stride must be at least dy_channels > 0");
+    if (x_channels <= 0 || x_stride < x_channels)
+      throw psv::ValueError("x_stride must be at least x_channels > 0");
+    if (ks != 1 && ks != 3) throw psv::ValueError("ks must be 1 or 3");
+    if (dy_gather2x2 && ks != 1) throw psv::ValueError("the 2x2 gather uses ks = 1");
+    if (batch <= 0 || h <= 0 || w <= 0) throw psv::ValueError("empty weight gradient");
+    if (workspace_floats < 0) throw psv::ValueError("negative workspace size");
+    psv::Wgrad a{};
+    a.l = (const uint16_t *)dy;
+    a.l_stride = dy_stride;
+    a.l_s2d = dy_gather2x2 ? 1 : 0;
+    a.lc = dy_channels;
+    a.I = dy_gather2x2 ? 4 * dy_channels : dy_channels;
+    a.r = (const uint16_t *)x;
+    a.r_stride = x_stride;
+    a.rc = x_channels;
+    a.ks = ks;
+    a.J = ks * ks * x_channels;
+    a.rows = batch * h;
+    a.H = h;
+    a.W = w;
+    hipError_t e = psv::launch_wgrad(a, dw, workspace, workspace ? workspace_floats : 0,
+                                     (hipStream_t)stream);
+    if (e != hipSuccess) throw psv::RuntimeError(std::string("wgrad: ") + hipGetErrorString(e));
+  });
+}
+
+int psv_channel_sum_bf16(const void *src, int stride, int offset, int channels,
+                         int64_t pixels, float *out, float *workspace,
+                         int64_t workspace_floats, void *stream) {
+  return guard([&] {
+    need(src, "src");
+    need(out, "out");
+    if (channels <= 0 || pixels <= 0 || offset < 0 || offset + channels > stride)
+      throw psv::ValueError("channel_sum needs 0 <= offset, offset + channels <= stride, "
+                            "pixels > 0");
+    hipError_t e = psv::launch_channel_sum((const uint16_t *)src, stride, offset, channels,
+                                           pixels, out, workspace,
+                                           workspace ? workspace_floats : 0,
+                                           (hipStream_t)stream);
+    if (e != hipSuccess)
+      throw psv::RuntimeError(std::string("channel_sum: ") + hipGetErrorString(e));
+  });
+}
+
+int psv_maxpool2_backward_bf16(const void *act, const void *z, const float *dpool,
+                               const void *dskip, int skip_stride, int skip_offset, void *out,
+                               int batch, int h, int w, int c, void *stream) {
+  return guard([&] {
+    need(z, "z");
+    need(out, "out");
+    if (dpool) need(act, "act");
+    if (batch <= 0 || h <= 0 || w <= 0 || h % 2 || w % 2 || c <= 0)
+      throw psv::ValueError("maxpool2_backward needs positive even h, w and c > 0");
+    if (dskip && (skip_offset < 0 || skip_offset + c > skip_stride))
+      throw psv::ValueError("skip channels exceed skip_stride");
+    hipError_t e = psv::launch_pool_backward(
+        (const uint16_t *)act, (const uint16_t *)z, dpool, (const uint16_t *)dskip,
+        skip_stride, skip_offset, (uint16_t *)out, batch * h, w, c, (hipStream_t)stream);
+    if (e != hipSuccess)
+      throw psv::RuntimeError(std::string("maxpool2_backward: ") + hipGetErrorString(e));
+  });
+}

@@ -230,11 +230,40 @@ struct BatchLaunch {
 hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
                              hipStream_t stream, hipEvent_t e0 = nullptr,
                              hipEvent_t e1 = nullptr);
-// Running sums of recorded receiver increments, in place: [nr][nt] x 2.
-// bfloat16 MFMA convolution for the preconditioner network (psv_conv.hip):
-// NHWC activations, weights [N][K], K = (ky * ks + kx) * cin + ci, cin a
-// multiple of 8; `transposed`: 2x2 stride-2 transposed convolution (ks = 1,
-// N = 4 cout).  Output bf16 or fp32 at channel offset out_off of rows of
+// bfloat16 MFMA kernels of the preconditioner network (psv_conv.hip).
+// NHWC bfloat16 activations with images stacked along the rows.
+enum ConvMode { kConv = 0, kScatter = 1, kGather2x2 = 2 };
+struct ConvGemm {
+  const uint16_t *in;   // gathered input, in_stride channels per pixel
+  const uint16_t *w;    // [N][K], K = taps * cin
+  const float *bias;    // [cout] or null
+  uint16_t *out16;      // bfloat16 output, or
+  float *out32;         // fp32 output
+  uint16_t *pre16;      // kConv only: pre-activation [pixels][cout], or null
+  const uint16_t *dz;   // multiply by gelu'(dz[pixel * dz_stride + co]), or null
+  int in_stride, out_stride, out_off, dz_stride;
+  int rows, H, W;       // rows = batch * H of the output (kScatter: of the input)
+  int cin, cout, ks, mode, gelu;
+};
+hipError_t launch_conv_gemm(const ConvGemm &a, hipStream_t s);
+// Weight gradient out[i][j] = sum_p L[p][i] R[p][j] over the rows x W
+// pixels p (see psv_conv.hip); fp32 result in dw, split partial sums in
+// workspace (fewer splits when it is small or null).
+struct Wgrad {
+  const uint16_t *l;
+  const uint16_t *r;
+  float *out;
+  long long chunk;
+  int l_stride, l_s2d, lc, r_stride, rc, ks, I, J, rows, H, W, nsplit;
+};
+hipError_t launch_wgrad(Wgrad a, float *dw, float *workspace, long long workspace_floats,
+                        hipStream_t s);
+hipError_t launch_channel_sum(const uint16_t *src, int stride, int off, int c, long long P,
+                              float *out, float *workspace, long long workspace_floats,
+                              hipStream_t s);
+// One image: cin a multiple of 8, weights [N][K], K = (ky * ks + kx) * cin +
+// ci; `transposed`: 2x2 stride-2 transposed convolution (ks = 1, N = 4
+// cout).  Output bf16 or fp32 at channel offset out_off of rows of
 // out_stride channels; optional fused erf GELU.
 hipError_t launch_conv_bf16(const uint16_t *in, const uint16_t *w, const float *bias,
                             void *out, bool out_f32, int h, int wd, int cin, int cout,
@@ -242,6 +271,10 @@ hipError_t launch_conv_bf16(const uint16_t *in, const uint16_t *w, const float *
                             int out_off, hipStream_t s);
 hipError_t launch_maxpool2_bf16(const uint16_t *in, uint16_t *out, int h, int w, int c,
                                 hipStream_t s);
+hipError_t launch_pool_backward(const uint16_t *act, const uint16_t *z, const float *dpool,
+                                const uint16_t *dskip, int skip_stride, int skip_off,
+                                uint16_t *out, int rows, int w, int c, hipStream_t s);
+// Running sums of recorded receiver increments, in place: [nr][nt] x 2.
 hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
                                 hipStream_t s);
 

@@ -428,3 +428,290 @@ def learned_direction_hip(unet, g, fields):
     """As learned_direction, with the network run by HipUNet."""
     y = unet(normalise_gradient(np.stack(g))).cpu().numpy().astype(np.float64)
     return [yy * f for yy, f in zip(y, fields)]
+
+
+class HipUNetTrainer:
+    """Training of a ``UNet`` with this library's bfloat16 MFMA kernels:
+    forward convolutions keep their bfloat16 pre-activations, the backward
+    pass runs input gradients as convolutions with flipped weights (GELU
+    derivative fused), weight gradients as pixel-reduction MFMA GEMMs and
+    the pooling/skip gradients in one fused kernel.  The loss and AdamW run
+    in torch on fp32 tensors (elementwise work on the small output and the
+    1.9 M parameters)."""
+
+    WORKSPACE_FLOATS = 1 << 25  # 128 MiB of split partial sums
+
+    def __init__(self, net, lr=2e-3, weight_decay=1e-4, epochs=20, device="cuda"):
+        import torch
+        from . import _lib
+        self.lib = _lib.load()
+        self.check = _lib.check
+        self.device = device
+        self.net = net.to(device)
+        self.opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
+        self.sched = torch.optim.lr_scheduler.CosineAnnealingLR(self.opt, T_max=epochs)
+        self.ws = torch.empty(self.WORKSPACE_FLOATS, device=device)
+
+    # ---- weight layouts -------------------------------------------------
+    @staticmethod
+    def _pad8(c):
+        return -(-c // 8) * 8
+
+    def _conv_layout(self, conv):
+        """Forward [co][(ky ks + kx) cin8 + ci] and input-gradient
+        [ci][(ky ks + kx) co8 + co] (flipped taps) weights, bfloat16."""
+        import torch
+        w = conv.weight.detach()
+        co, ci, k, _ = w.shape
+        c8, o8 = self._pad8(ci), self._pad8(co)
+        f = torch.zeros(co, k, k, c8, device=self.device)
+        f[..., :ci] = w.permute(0, 2, 3, 1)
+        d = torch.zeros(ci, k, k, o8, device=self.device)
+        d[..., :co] = w.flip(2, 3).permute(1, 2, 3, 0)
+        return dict(kind="conv", mod=conv, ci=ci, c8=c8, co=co, o8=o8, k=k,
+                    wf=f.reshape(co, -1).to(torch.bfloat16).contiguous(),
+                    wd=d.reshape(ci, -1).to(torch.bfloat16).contiguous(),
+                    b=conv.bias.detach().float().contiguous())
+
+    def _convt_layout(self, ct):
+        import torch
+        w = ct.weight.detach()  # (ci, co, 2, 2)
+        ci, co = w.shape[:2]
+        return dict(kind="convt", mod=ct, ci=ci, co=co,
+                    wf=w.permute(2, 3, 1, 0).reshape(4 * co, ci).to(torch.bfloat16).contiguous(),
+                    wd=w.permute(0, 2, 3, 1).reshape(ci, 4 * co).to(torch.bfloat16).contiguous(),
+                    b=ct.bias.detach().float().contiguous())
+
+    def _layouts(self):
+        n = self.net
+        return dict(enc=[(self._conv_layout(b[0]), self._conv_layout(b[2])) for b in n.enc],
+                    mid=(self._conv_layout(n.mid[0]), self._conv_layout(n.mid[2])),
+                    up=[self._convt_layout(u) for u in n.up],
+                    dec=[(self._conv_layout(b[0]), self._conv_layout(b[2])) for b in n.dec],
+                    out=self._conv_layout(n.out))
+
+    # ---- kernel calls ---------------------------------------------------
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+
+    def _empty(self, *shape, f32=False):
+        import torch
+        return torch.empty(*shape, device=self.device,
+                           dtype=torch.float32 if f32 else torch.bfloat16)
+
+    def _conv(self, x, in_stride, cin, wt, bias, out, out_stride, out_off, cout, bhw, ks=3,
+              mode=0, gelu=False, pre=None, dz=None, dz_stride=0, f32=False):
+        b, h, w = bhw
+        self.check(self.lib.psv_conv_bf16(
+            x.data_ptr(), in_stride, wt.data_ptr(), 0 if bias is None else bias.data_ptr(),
+            out.data_ptr(), int(f32), out_stride, out_off,
+            0 if pre is None else pre.data_ptr(), 0 if dz is None else dz.data_ptr(),
+            dz_stride, b, h, w, cin, cout, ks, mode, int(gelu), self._stream()))
+        return out
+
+    def _wgrad(self, dy, dy_stride, gather, dy_ch, x, x_stride, x_ch, ks, bhw, dw):
+        b, h, w = bhw
+        self.check(self.lib.psv_conv_wgrad_bf16(
+            dy.data_ptr(), dy_stride, int(gather), dy_ch, x.data_ptr(), x_stride, x_ch, ks,
+            b, h, w, dw.data_ptr(), self.ws.data_ptr(), self.ws.numel(), self._stream()))
+        return dw
+
+    def _chsum(self, src, stride, off, ch, pixels):
+        out = self._empty(ch, f32=True)
+        self.check(self.lib.psv_channel_sum_bf16(
+            src.data_ptr(), stride, off, ch, pixels, out.data_ptr(), self.ws.data_ptr(),
+            self.ws.numel(), self._stream()))
+        return out
+
+    # ---- network --------------------------------------------------------
+    def _block_forward(self, x, cin_stride, cin, L, bhw):
+        c1, c2 = L
+        b, h, w = bhw
+        z1, a1 = self._empty(b, h, w, c1["co"]), self._empty(b, h, w, c1["co"])
+        self._conv(x, cin_stride, cin, c1["wf"], c1["b"], a1, c1["co"], 0, c1["co"], bhw,
+                   gelu=True, pre=z1)
+        z2, a2 = self._empty(b, h, w, c2["co"]), self._empty(b, h, w, c2["co"])
+        self._conv(a1, c1["co"], c1["co"], c2["wf"], c2["b"], a2, c2["co"], 0, c2["co"], bhw,
+                   gelu=True, pre=z2)
+        return dict(x=x, xs=cin_stride, xc=cin, z1=z1, a1=a1, z2=z2, a2=a2, bhw=bhw)
+
+    def forward(self, x, L, keep=True):
+        """x: (B, h, w, 8) bfloat16 NHWC input (h, w multiples of 16).
+        Returns the fp32 (B, h, w, 3) output and the saved activations."""
+        b, h, w, c8 = x.shape
+        saved = dict(enc=[], dec=[], up_in=[], cat=[])
+        a, stride, ch = x, c8, c8
+        for lvl in range(3):
+            s = self._block_forward(a, stride, ch, L["enc"][lvl], (b, h, w))
+            saved["enc"].append(s)
+            c = L["enc"][lvl][1]["co"]
+            pooled = self._empty(b, h // 2, w // 2, c)
+            self.check(self.lib.psv_maxpool2_bf16(s["a2"].data_ptr(), pooled.data_ptr(),
+                                                  b * h, w, c, self._stream()))
+            a, stride, ch, h, w = pooled, c, c, h // 2, w // 2
+        s = self._block_forward(a, stride, ch, L["mid"], (b, h, w))
+        saved["mid"] = s
+        a = s["a2"]
+        for lvl in range(3):
+            up, skip = L["up"][lvl], saved["enc"][2 - lvl]
+            cu, cs = up["co"], skip["a2"].shape[-1]
+            cat = self._empty(b, 2 * h, 2 * w, cu + cs)
+            self._conv(a, up["ci"], up["ci"], up["wf"], up["b"], cat, cu + cs, 0, cu,
+                       (b, h, w), ks=1, mode=1)
+            cat[..., cu:] = skip["a2"]
+            saved["up_in"].append(a)
+            saved["cat"].append(cat)
+            h, w = 2 * h, 2 * w
+            s = self._block_forward(cat, cu + cs, cu + cs, L["dec"][lvl], (b, h, w))
+            saved["dec"].append(s)
+            a = s["a2"]
+        o = L["out"]
+        y = self._empty(b, h, w, o["co"], f32=True)
+        self._conv(a, o["ci"], o["ci"], o["wf"], o["b"], y, o["co"], 0, o["co"], (b, h, w),
+                   ks=1, f32=True)
+        return y, saved
+
+    def _block_backward(self, s, L, dz2, grads, dx_f32=False, need_dx=True):
+        """dz2: gradient at the block's second pre-activation.  Adds the
+        weight/bias gradients to `grads`; returns the gradient at the block
+        input (fp32 or bf16), or None."""
+        c1, c2 = L
+        b, h, w = s["bhw"]
+        P = b * h * w
+        dw2 = self._empty(c2["co"], 9 * c2["c8"], f32=True)
+        self._wgrad(dz2, c2["co"], 0, c2["co"], s["a1"], c1["co"], c2["c8"], 3, s["bhw"], dw2)
+        grads[c2["mod"]] = (dw2, self._chsum(dz2, c2["co"], 0, c2["co"], P))
+        dz1 = self._empty(b, h, w, c1["co"])
+        self._conv(dz2, c2["co"], c2["o8"], c2["wd"], None, dz1, c1["co"], 0, c2["ci"],
+                   s["bhw"], dz=s["z1"], dz_stride=c1["co"])
+        dw1 = self._empty(c1["co"], 9 * c1["c8"], f32=True)
+        self._wgrad(dz1, c1["co"], 0, c1["co"], s["x"], s["xs"], c1["c8"], 3, s["bhw"], dw1)
+        grads[c1["mod"]] = (dw1, self._chsum(dz1, c1["co"], 0, c1["co"], P))
+        if not need_dx:
+            return None
+        dx = self._empty(b, h, w, c1["ci"], f32=dx_f32)
+        self._conv(dz1, c1["co"], c1["o8"], c1["wd"], None, dx, c1["ci"], 0, c1["ci"],
+                   s["bhw"], f32=dx_f32)
+        return dx
+
+    def backward(self, dy, saved, L):
+        """dy: fp32 (B, h, w, 3) gradient at the output.  Returns
+        {module: (dW in kernel layout, db)}."""
+        import torch
+        grads = {}
+        b, h, w, _ = dy.shape
+        o = L["out"]
+        dyo = torch.zeros(b, h, w, o["o8"], device=self.device, dtype=torch.bfloat16)
+        dyo[..., :o["co"]] = dy
+        last = saved["dec"][2]
+        dwo = self._empty(o["co"], o["c8"], f32=True)
+        self._wgrad(dyo, o["o8"], 0, o["co"], last["a2"], o["ci"], o["c8"], 1, (b, h, w), dwo)
+        grads[o["mod"]] = (dwo, self._chsum(dyo, o["o8"], 0, o["co"], b * h * w))
+        dz2 = self._empty(b, h, w, o["ci"])
+        self._conv(dyo, o["o8"], o["o8"], o["wd"], None, dz2, o["ci"], 0, o["ci"], (b, h, w),
+                   ks=1, dz=last["z2"], dz_stride=o["ci"])
+        dskips = [None] * 3
+        for lvl in (2, 1, 0):
+            s, up = saved["dec"][lvl], L["up"][lvl]
+            dcat = self._block_backward(s, L["dec"][lvl], dz2, grads)
+            bb, hh, ww = s["bhw"]
+            cu, ccat = up["co"], dcat.shape[-1]
+            dskips[2 - lvl] = (dcat, ccat, cu)
+            lo = (bb, hh // 2, ww // 2)
+            dwu = self._empty(4 * cu, up["ci"], f32=True)
+            self._wgrad(dcat, ccat, 1, cu, saved["up_in"][lvl], up["ci"], up["ci"], 1, lo, dwu)
+            grads[up["mod"]] = (dwu, self._chsum(dcat, ccat, 0, cu, bb * hh * ww))
+            prev = saved["mid"] if lvl == 0 else saved["dec"][lvl - 1]
+            dz2 = self._empty(bb, hh // 2, ww // 2, up["ci"])
+            self._conv(dcat, ccat, cu, up["wd"], None, dz2, up["ci"], 0, up["ci"], lo, ks=1,
+                       mode=2, dz=prev["z2"], dz_stride=up["ci"])
+        dpool = self._block_backward(saved["mid"], L["mid"], dz2, grads, dx_f32=True)
+        for lvl in (2, 1, 0):
+            s = saved["enc"][lvl]
+            bb, hh, ww = s["bhw"]
+            c = s["a2"].shape[-1]
+            dcat, ccat, cu = dskips[lvl]
+            dz2 = self._empty(bb, hh, ww, c)
+            self.check(self.lib.psv_maxpool2_backward_bf16(
+                s["a2"].data_ptr(), s["z2"].data_ptr(), dpool.data_ptr(), dcat.data_ptr(),
+                ccat, cu, dz2.data_ptr(), bb, hh, ww, c, self._stream()))
+            dpool = self._block_backward(s, L["enc"][lvl], dz2, grads, dx_f32=True,
+                                         need_dx=lvl > 0)
+        return grads
+
+    @staticmethod
+    def _to_param_grads(L, grads):
+        """Kernel-layout gradients -> (module, dW shaped like the weight, db)."""
+        out = []
+        for mod, (dw, db) in grads.items():
+            wsh = mod.weight.shape
+            if isinstance(mod, type(L["up"][0]["mod"])):
+                ci, co = wsh[:2]
+                g = dw.view(2, 2, co, ci).permute(3, 2, 0, 1)
+            else:
+                co, ci, k, _ = wsh
+                g = dw.view(co, k, k, -1)[..., :ci].permute(0, 3, 1, 2)
+            out.append((mod, g.contiguous(), db))
+        return out
+
+    def prepare(self, x):
+        """(B, 3, nx, nz) fp32 tensor -> (B, h, w, 8) bfloat16 NHWC padded."""
+        import torch
+        b, _, nx, nz = x.shape
+        h, w = nx + (-nx) % 16, nz + (-nz) % 16
+        xx = torch.zeros(b, h, w, 8, device=self.device, dtype=torch.bfloat16)
+        xx[:, :nx, :nz, :3] = x.permute(0, 2, 3, 1)
+        return xx
+
+    def loss_and_grads(self, x, y, mask):
+        """Loss of train() and the parameter gradients (module, dW, db)."""
+        import torch
+        L = self._layouts()
+        nx, nz = x.shape[-2:]
+        out, saved = self.forward(self.prepare(x), L)
+        pred = out[:, :nx, :nz, :].permute(0, 3, 1, 2).detach().requires_grad_(True)
+        loss = (((pred - y) * mask) ** 2).mean() / (mask.mean() * 1e-3)
+        loss.backward()
+        dy = torch.zeros_like(out)
+        dy[:, :nx, :nz, :] = pred.grad.permute(0, 2, 3, 1)
+        grads = self.backward(dy, saved, L)
+        return loss.detach(), self._to_param_grads(L, grads)
+
+    def step(self, x, y, mask):
+        """One AdamW step on a batch; returns the loss (a tensor)."""
+        loss, grads = self.loss_and_grads(x, y, mask)
+        for mod, gw, gb in grads:
+            mod.weight.grad = gw
+            mod.bias.grad = gb
+        self.opt.step()
+        return loss
+
+
+def train_hip(net, grads, perts, epochs=20, batch=16, lr=2e-3, device="cuda", log=None):
+    """train() with the forward and backward passes on HipUNetTrainer's
+    kernels (same data order, augmentation, loss and optimiser)."""
+    import torch
+    tr = HipUNetTrainer(net, lr=lr, epochs=epochs, device=device)
+    mask = torch.tensor(kernel_mask(grads.shape[-2:]), dtype=torch.float32, device=device)
+    x_all = torch.tensor(normalise_gradient(grads), device=device)
+    y_all = torch.tensor(perts, device=device)
+    n = x_all.shape[0]
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    history = []
+    for ep in range(epochs):
+        te = time.perf_counter()
+        perm = torch.randperm(n, generator=gen)
+        tot = torch.zeros((), device=device)
+        for i in range(0, n, batch):
+            idx = perm[i:i + batch].to(device)
+            x, y = x_all[idx], y_all[idx]
+            if torch.rand(1, generator=gen).item() < 0.5:  # mirror in x
+                x, y = torch.flip(x, dims=[2]), torch.flip(y, dims=[2])
+            tot += tr.step(x, y, mask) * x.shape[0]
+        tr.sched.step()
+        history.append(float(tot) / n)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {history[-1]:.5f} "
+                f"({time.perf_counter() - te:.1f} s)")
+    return history
