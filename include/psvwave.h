@@ -260,9 +260,10 @@ int psv_batch_timing(psv_batch *b, double *loop_ms, int64_t *points,
  * caller owns, enqueued on `stream` (a hipStream_t; NULL = the default
  * stream) and not synchronised.  Activations NHWC (one image) as bfloat16
  * bit patterns; weights [N][K] with K = (ky * ks + kx) * cin + ci; cin a
- * multiple of 8; ks 1 or 3 ("same" padding).  transposed = 1: 2x2
- * stride-2 transposed convolution (ks = 1, N = 4 * cout, n = (2a + b) *
- * cout + co writes pixel (2y + a, 2x + b) of a (2h, 2w) output).  The
+ * multiple of 8; in and weights 16-byte aligned; ks 1 or 3 ("same"
+ * padding).  transposed = 1: 2x2 stride-2 transposed convolution (ks = 1,
+ * N = 4 * cout, n = (2a + b) * cout + co writes pixel (2y + a, 2x + b) of
+ * a (2h, 2w) output).  The
  * result (+ bias, then erf GELU if gelu = 1) goes to channel out_offset of
  * rows of out_stride channels, fp32 if out_f32 = 1 else bfloat16. */
 int psv_conv2d_bf16(const void *in, const void *weights, const float *bias,
@@ -277,7 +278,8 @@ int psv_maxpool2_bf16(const void *in, void *out, int h, int w, int c,
 /* Training kernels of the same network (bfloat16 operands, fp32
  * accumulation).  Images are stacked along the rows: `batch` images of
  * h x w pixels.  General convolution: the input has in_stride channels per
- * pixel, of which the first cin (a multiple of 8) are read.
+ * pixel, of which the first cin (a multiple of 8) are read; in and weights
+ * are 16-byte aligned.
  *   mode 0: ks x ks convolution (ks 1 or 3, "same" padding per image),
  *           K = ks * ks * cin, output h x w;
  *   mode 1: 2x2 stride-2 transposed convolution of an h x w input
@@ -297,13 +299,18 @@ int psv_conv_bf16(const void *in, int in_stride, const void *weights, const floa
  * x_channels + ci] = sum over pixels of dy[p][co] x(p shifted by the tap)[ci]
  * (dy_gather2x2 = 0, dy at the h x w pixels), or of a mode-1 transposed
  * convolution, dw[(2a + b) * dy_channels + co][ci] (dy_gather2x2 = 1: dy at
- * 2h x 2w pixels, ks = 1).  fp32 result; the pixel sum is split over
- * workspace (up to workspace_floats floats; NULL = no split) and added in a
- * fixed order, so the result is deterministic. */
+ * 2h x 2w pixels, ks = 1).  dy_stride, x_channels and x_stride are
+ * multiples of 8 (and dy_channels for the gather); dy and x are 16-byte
+ * aligned.  db (may be NULL) receives the pixel sums of
+ * dy's gathered columns: the bias gradient of a convolution ([co]); for the
+ * 2x2 gather [(2a + b) * dy_channels + co], whose four positions add up to
+ * the bias gradient.  fp32 results; the pixel sum is split over workspace
+ * (up to workspace_floats floats; NULL = no split) and added in a fixed
+ * order, so the result is deterministic. */
 int psv_conv_wgrad_bf16(const void *dy, int dy_stride, int dy_gather2x2, int dy_channels,
                         const void *x, int x_stride, int x_channels, int ks, int batch, int h,
-                        int w, float *dw, float *workspace, int64_t workspace_floats,
-                        void *stream);
+                        int w, float *dw, float *db, float *workspace,
+                        int64_t workspace_floats, void *stream);
 /* out[c] = sum over pixels of src[p * stride + offset + c] (bias gradients),
  * fixed-order fp32 sums. */
 int psv_channel_sum_bf16(const void *src, int stride, int offset, int channels,

@@ -132,7 +132,7 @@ _SIGNATURES = {
     "psv_conv_bf16": [C.c_void_p, _i, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i, _i,
                       C.c_void_p, C.c_void_p, _i, _i, _i, _i, _i, _i, _i, _i, _i, C.c_void_p],
     "psv_conv_wgrad_bf16": [C.c_void_p, _i, _i, _i, C.c_void_p, _i, _i, _i, _i, _i, _i,
-                            C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
+                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p],
     "psv_channel_sum_bf16": [C.c_void_p, _i, _i, _i, C.c_int64, C.c_void_p, C.c_void_p,
                              C.c_int64, C.c_void_p],
     "psv_maxpool2_backward_bf16": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i, _i,

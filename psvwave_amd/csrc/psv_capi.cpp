@@ -735,6 +735,8 @@ int psv_conv2d_bf16(const void *in, const void *weights, const float *bias, void
     need(bias, "bias");
     need(out, "out");
     if (cin <= 0 || cin % 8 != 0) throw psv::ValueError("cin must be a positive multiple of 8");
+    if (((uintptr_t)in | (uintptr_t)weights) & 15)
+      throw psv::ValueError("in and weights must be 16-byte aligned");
     if (ks != 1 && ks != 3) throw psv::ValueError("ks must be 1 or 3");
     if (transposed && ks != 1) throw psv::ValueError("transposed convolutions use ks = 1");
     if (h <= 0 || w <= 0 || cout <= 0) throw psv::ValueError("empty convolution");
@@ -772,6 +774,8 @@ int psv_conv_bf16(const void *in, int in_stride, const void *weights, const floa
     if (cin <= 0 || cin % 8 != 0) throw psv::ValueError("cin must be a positive multiple of 8");
     if (in_stride < cin || in_stride % 8 != 0)
       throw psv::ValueError("in_stride must be a multiple of 8 and at least cin");
+    if (((uintptr_t)in | (uintptr_t)weights) & 15)
+      throw psv::ValueError("in and weights must be 16-byte aligned");
     if (mode == psv::kConv ? (ks != 1 && ks != 3) : ks != 1)
       throw psv::ValueError("ks must be 1 or 3 (1 for modes 1 and 2)");
     if (batch <= 0 || h <= 0 || w <= 0 || cout <= 0) throw psv::ValueError("empty convolution");
@@ -807,16 +811,21 @@ int psv_conv_bf16(const void *in, int in_stride, const void *weights, const floa
 
 int psv_conv_wgrad_bf16(const void *dy, int dy_stride, int dy_gather2x2, int dy_channels,
                         const void *x, int x_stride, int x_channels, int ks, int batch, int h,
-                        int w, float *dw, float *workspace, int64_t workspace_floats,
-                        void *stream) {
+                        int w, float *dw, float *db, float *workspace,
+                        int64_t workspace_floats, void *stream) {
   return guard([&] {
     need(dy, "dy");
     need(x, "x");
     need(dw, "dw");
-    if (dy_channels <= 0 || dy_stride < dy_channels)
-      throw psv::ValueError("dy_stride must be at least dy_channels > 0");
-    if (x_channels <= 0 || x_stride < x_channels)
-      throw psv::ValueError("x_stride must be at least x_channels > 0");
+    if (dy_channels <= 0 || dy_stride < dy_channels || dy_stride % 8)
+      throw psv::ValueError("dy_stride must be a multiple of 8 and at least dy_channels > 0");
+    if (x_channels <= 0 || x_channels % 8 || x_stride < x_channels || x_stride % 8)
+      throw psv::ValueError("x_channels and x_stride must be multiples of 8, x_stride >= "
+                            "x_channels");
+    if (dy_gather2x2 && dy_channels % 8)
+      throw psv::ValueError("the 2x2 gather needs dy_channels a multiple of 8");
+    if (((uintptr_t)dy | (uintptr_t)x) & 15)
+      throw psv::ValueError("dy and x must be 16-byte aligned");
     if (ks != 1 && ks != 3) throw psv::ValueError("ks must be 1 or 3");
     if (dy_gather2x2 && ks != 1) throw psv::ValueError("the 2x2 gather uses ks = 1");
     if (batch <= 0 || h <= 0 || w <= 0) throw psv::ValueError("empty weight gradient");
@@ -835,7 +844,7 @@ int psv_conv_wgrad_bf16(const void *dy, int dy_stride, int dy_gather2x2, int dy_
     a.rows = batch * h;
     a.H = h;
     a.W = w;
-    hipError_t e = psv::launch_wgrad(a, dw, workspace, workspace ? workspace_floats : 0,
+    hipError_t e = psv::launch_wgrad(a, dw, db, workspace, workspace ? workspace_floats : 0,
                                      (hipStream_t)stream);
     if (e != hipSuccess) throw psv::RuntimeError(std::string("wgrad: ") + hipGetErrorString(e));
   });

@@ -11,8 +11,8 @@
 //   kGather2x2  space-to-depth gather (the input gradient of the transposed
 //               convolution): the input has 2H x 2W pixels,
 //               k = q * cin + ci reads pixel (2y + q / 2, 2x + q % 2).
-// One wavefront computes a 32 x 32 tile of out with v_mfma_f32_32x32x8_bf16
-// (fp32 accumulation); bias, the erf GELU (optionally storing the
+// One wavefront computes a 64 x 64 (or 128 x 32) tile of out with
+// v_mfma_f32_32x32x16_bf16 (fp32 accumulation); bias, the erf GELU (optionally storing the
 // pre-activation) and the GELU derivative of a stored pre-activation (the
 // backward pass) are fused into the epilogue.
 //
@@ -30,6 +30,8 @@ namespace psv {
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ inline uint16_t to_bf16(float v) {
@@ -53,66 +55,120 @@ __device__ inline float gelu_grad(float v) {
 // 8 (r / 4) + 4 half + r % 4, column l32 of the 32 x 32 tile.
 __device__ inline int acc_row(int r, int half) { return 8 * (r >> 2) + 4 * half + (r & 3); }
 
-// The input channel count (cin) and in_stride are multiples of 8, so each
-// 8-wide step of K lies in one tap and a lane's 4 values are 8-byte aligned.
+// A lane's 8 values of K (k = k0 + 8 half ... + 7) lie in one tap because
+// cin and in_stride are multiples of 8, so they are one aligned 16-byte load.
+// Source element offset of pixel (yg, x) (yi = row within its image) for a
+// tap, or -1 outside the image.
+__device__ inline long long conv_src(const ConvGemm &a, int tap, int yg, int x, int yi) {
+  if (a.mode == kGather2x2)
+    return ((long long)(2 * yg + (tap >> 1)) * (2 * a.W) + 2 * x + (tap & 1)) * a.in_stride;
+  const int pad = a.ks / 2, dy = tap / a.ks - pad, dx = tap % a.ks - pad;
+  if (yi + dy < 0 || yi + dy >= a.H || x + dx < 0 || x + dx >= a.W) return -1;
+  return ((long long)(yg + dy) * a.W + x + dx) * a.in_stride;
+}
+
+// One wavefront computes a (32 WM) x (32 WN) tile of out with
+// v_mfma_f32_32x32x16_bf16: per step of 16 along K each lane loads 16 bytes
+// per 32-row slice of A and of B, and issues WM x WN matrix instructions.
+template <int WM, int WN>
 __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvGemm a) {
   const int lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
   const int M = a.rows * a.W, N = a.mode == kScatter ? 4 * a.cout : a.cout;
   const int ntaps = a.mode == kGather2x2 ? 4 : a.ks * a.ks, K = ntaps * a.cin;
-  const int ntm = (M + 31) / 32, ntn = (N + 31) / 32;
+  const int ntm = (M + 32 * WM - 1) / (32 * WM), ntn = (N + 32 * WN - 1) / (32 * WN);
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= ntm * ntn) return;  // whole wavefronts exit together
   const int tm = tile % ntm, tn = tile / ntm;
-  const int m = tm * 32 + l32, n = tn * 32 + l32;
-  const int yg = m / a.W, x = m - yg * a.W, yi = yg % a.H, pad = a.ks / 2;
-  const bool mok = m < M, nok = n < N;
-  f32x16 acc;
+  int yg[WM], xx[WM], yi[WM];
+  bool mok[WM];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (int tap = 0; tap < ntaps; ++tap) {
-    long long src;
-    bool ok = mok;
-    if (a.mode == kGather2x2) {
-      src = (long long)(2 * yg + (tap >> 1)) * (2 * a.W) + 2 * x + (tap & 1);
-    } else {
-      const int dy = tap / a.ks - pad, dx = tap % a.ks - pad;
-      ok = ok && yi + dy >= 0 && yi + dy < a.H && x + dx >= 0 && x + dx < a.W;
-      src = (long long)(yg + dy) * a.W + x + dx;
-    }
-    const uint16_t *ap = a.in + (ok ? src * a.in_stride : 0) + 4 * half;
-    const uint16_t *bp = a.w + (nok ? (long long)n * K : 0) + tap * a.cin + 4 * half;
-    for (int c0 = 0; c0 < a.cin; c0 += 8) {
-      bf16x4 av = {0, 0, 0, 0}, bv = {0, 0, 0, 0};
-      if (ok) av = *(const bf16x4 *)(ap + c0);
-      if (nok) bv = *(const bf16x4 *)(bp + c0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  for (int i = 0; i < WM; ++i) {
+    const int m = (tm * WM + i) * 32 + l32;
+    mok[i] = m < M;
+    yg[i] = m / a.W;
+    xx[i] = m - yg[i] * a.W;
+    yi[i] = yg[i] % a.H;
+  }
+  const uint16_t *bp[WN];
+  bool nok[WN];
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = (tn * WN + j) * 32 + l32;
+    nok[j] = n < N;
+    bp[j] = a.w + (nok[j] ? (long long)n * K : 0) + 8 * half;
+  }
+  // this lane's position along K: tap and channel of k0 + 8 half
+  int tap = (8 * half) / a.cin, ci = 8 * half - tap * a.cin;
+  long long src[WM];
+#pragma unroll
+  for (int i = 0; i < WM; ++i) src[i] = mok[i] ? conv_src(a, tap, yg[i], xx[i], yi[i]) : -1;
+  f32x16 acc[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  const bf16x8 zero = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const bool kok = tap < ntaps;
+    bf16x8 av[WM], bv[WN];
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+      av[i] = kok && src[i] >= 0 ? *(const bf16x8 *)(a.in + src[i] + ci) : zero;
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+      bv[j] = kok && nok[j] ? *(const bf16x8 *)(bp[j] + k0) : zero;
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+    ci += 16;
+    if (ci >= a.cin) {
+      do {
+        ci -= a.cin;
+        ++tap;
+      } while (ci >= a.cin);
+      if (tap < ntaps) {
+#pragma unroll
+        for (int i = 0; i < WM; ++i)
+          src[i] = mok[i] ? conv_src(a, tap, yg[i], xx[i], yi[i]) : -1;
+      }
     }
   }
-  if (!nok) return;
-  const int co = a.mode == kScatter ? n % a.cout : n;
-  const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int mm = tm * 32 + acc_row(r, half);
-    if (mm >= M) continue;
-    long long pix = mm;
-    if (a.mode == kScatter) {
-      const int q = n / a.cout, py = mm / a.W, px = mm - py * a.W;
-      pix = (long long)(2 * py + (q >> 1)) * (2 * a.W) + (2 * px + (q & 1));
+  for (int j = 0; j < WN; ++j) {
+    if (!nok[j]) continue;
+    const int n = (tn * WN + j) * 32 + l32;
+    const int co = a.mode == kScatter ? n % a.cout : n;
+    const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int mm = (tm * WM + i) * 32 + acc_row(r, half);
+        if (mm >= M) continue;
+        long long pix = mm;
+        if (a.mode == kScatter) {
+          const int q = n / a.cout, py = mm / a.W, px = mm - py * a.W;
+          pix = (long long)(2 * py + (q >> 1)) * (2 * a.W) + (2 * px + (q & 1));
+        }
+        float v = acc[i][j][r] + b;
+        if (a.pre16) {  // training: keep the rounded pre-activation for the backward pass
+          const uint16_t h = to_bf16(v);
+          a.pre16[pix * a.cout + co] = h;
+          v = from_bf16(h);
+        }
+        if (a.gelu) v = gelu(v);
+        if (a.dz) v *= gelu_grad(from_bf16(a.dz[pix * a.dz_stride + co]));
+        const long long o = pix * a.out_stride + a.out_off + co;
+        if (a.out32)
+          a.out32[o] = v;
+        else
+          a.out16[o] = to_bf16(v);
+      }
     }
-    float v = acc[r] + b;
-    if (a.pre16) {  // training: keep the rounded pre-activation for the backward pass
-      const uint16_t h = to_bf16(v);
-      a.pre16[pix * a.cout + co] = h;
-      v = from_bf16(h);
-    }
-    if (a.gelu) v = gelu(v);
-    if (a.dz) v *= gelu_grad(from_bf16(a.dz[pix * a.dz_stride + co]));
-    const long long o = pix * a.out_stride + a.out_off + co;
-    if (a.out32)
-      a.out32[o] = v;
-    else
-      a.out16[o] = to_bf16(v);
   }
 }
 
@@ -121,85 +177,159 @@ __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvGemm a) {
 // (l_s2d = 0: L[p][i] = l[p * l_stride + i]; l_s2d = 1: i = q * lc + c reads
 // fine pixel (2y + q / 2, 2x + q % 2)); R: the layer input gathered as the
 // forward convolution gathers it (j = tap * rc + c, zero outside the image).
+// One wavefront computes a 64 x 64 tile of out.  Per step of 16 pixels each
+// lane loads 16 bytes (8 consecutive channels of one pixel) from two
+// 8-column chunks of L and of R, and writes them transposed into this
+// wavefront's part of LDS ([column][pixel]); the MFMA fragments
+// (8 consecutive pixels of one column) are then single 16-byte LDS reads.
+constexpr int kWgRow = 24;  // LDS row pitch in bf16 (16 pixels + 8 pad): 16-byte aligned rows
 __global__ void __launch_bounds__(256) wgrad_mfma_kernel(Wgrad a) {
-  const int lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
-  const int nti = (a.I + 31) / 32, ntj = (a.J + 31) / 32;
-  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (tile >= nti * ntj * a.nsplit) return;
+  __shared__ __attribute__((aligned(16))) uint16_t lds[4][2][64 * kWgRow];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, half = lane >> 5, l32 = lane & 31;
+  const int nti = (a.I + 63) / 64, ntj = (a.J + 63) / 64;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= nti * ntj * a.nsplit) return;  // whole wavefronts exit together
   const int ti = tile % nti, tj = (tile / nti) % ntj, s = tile / (nti * ntj);
-  const int i = ti * 32 + l32, j = tj * 32 + l32;
-  const bool iok = i < a.I, jok = j < a.J;
+  const int i0 = ti * 64, j0 = tj * 64;
   const long long P = (long long)a.rows * a.W;
   const long long p_begin = (long long)s * a.chunk;
   const long long p_end = p_begin + a.chunk < P ? p_begin + a.chunk : P;
-  // per-lane constants of the two gathers
-  int lq = 0, lci = i;
-  if (a.l_s2d) {
-    lq = i / a.lc;
-    lci = i - lq * a.lc;
+  uint16_t *la = lds[wave][0], *lb = lds[wave][1];
+  // loading role: pixel slot ps of each step, column chunks 2 cg and 2 cg + 1
+  const int ps = lane >> 2, cg = lane & 3;
+  int lq[2], lcol[2], dy[2], dx[2], rcol[2];
+  bool lok[2], rok[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int i = i0 + 8 * (2 * cg + c), j = j0 + 8 * (2 * cg + c);
+    lok[c] = i < a.I;
+    lq[c] = a.l_s2d ? i / a.lc : 0;
+    lcol[c] = a.l_s2d ? i - lq[c] * a.lc : i;
+    rok[c] = j < a.J;
+    const int tap = j / a.rc, pad = a.ks / 2;
+    rcol[c] = j - tap * a.rc;
+    dy[c] = tap / a.ks - pad;
+    dx[c] = tap % a.ks - pad;
   }
-  const int tap = j / a.rc, rci = j - tap * a.rc, pad = a.ks / 2;
-  const int dy = tap / a.ks - pad, dx = tap % a.ks - pad;
-  // the 4 pixels this lane feeds per step: p0 + 4 half + jj
-  int px[4], pyg[4], pyi[4];
+  long long p = p_begin + ps;
+  int yg = (int)(p / a.W), x = (int)(p - (long long)yg * a.W), yi = yg % a.H;
+  f32x16 acc[2][2];
 #pragma unroll
-  for (int jj = 0; jj < 4; ++jj) {
-    const long long p = p_begin + 4 * half + jj;
-    pyg[jj] = (int)(p / a.W);
-    px[jj] = (int)(p - (long long)pyg[jj] * a.W);
-    pyi[jj] = pyg[jj] % a.H;
-  }
-  f32x16 acc;
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (long long p0 = p_begin; p0 < p_end; p0 += 8) {
-    bf16x4 av, bv;
+    for (int v = 0; v < 2; ++v)
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      const long long p = p0 + 4 * half + jj;
-      const bool pok = p < p_end;
-      uint16_t lv = 0, rv = 0;
-      if (pok && iok) {
+      for (int r = 0; r < 16; ++r) acc[u][v][r] = 0.f;
+  const bool bias_tile = a.bias_out && tj == 0;  // also sums L over the pixels
+  float lsum = 0.f;
+  const u32x4 zero = {0u, 0u, 0u, 0u};
+  for (long long p0 = p_begin; p0 < p_end; p0 += 16, p += 16) {
+    const bool pok = p < p_end;
+    u32x4 lv[2], rv[2];  // 8 bf16 each, element t in the (t & 1) half of word t / 2
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      lv[c] = zero;
+      if (pok && lok[c]) {
         long long li;
         if (a.l_s2d)
-          li = ((long long)(2 * pyg[jj] + (lq >> 1)) * (2 * a.W) + 2 * px[jj] + (lq & 1)) *
-                   a.l_stride + lci;
+          li = ((long long)(2 * yg + (lq[c] >> 1)) * (2 * a.W) + 2 * x + (lq[c] & 1)) *
+                   a.l_stride + lcol[c];
         else
-          li = p * a.l_stride + i;
-        lv = a.l[li];
+          li = p * a.l_stride + lcol[c];
+        lv[c] = *(const u32x4 *)(a.l + li);
       }
-      const int yy = pyi[jj] + dy, xx = px[jj] + dx;
-      if (pok && jok && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-        rv = a.r[((long long)(pyg[jj] + dy) * a.W + xx) * a.r_stride + rci];
-      av[jj] = (short)lv;
-      bv[jj] = (short)rv;
-      // advance this pixel by 8
-      px[jj] += 8;
-      while (px[jj] >= a.W) {
-        px[jj] -= a.W;
-        ++pyg[jj];
-        if (++pyi[jj] == a.H) pyi[jj] = 0;
-      }
+      rv[c] = zero;
+      const int yy = yi + dy[c], xx = x + dx[c];
+      if (pok && rok[c] && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+        rv[c] = *(const u32x4 *)(a.r + ((long long)(yg + dy[c]) * a.W + xx) * a.r_stride +
+                                 rcol[c]);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+    // transpose into LDS: column 8 (2 cg + c) + t, pixel ps
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int row = 8 * (2 * cg + c) + t;
+        la[row * kWgRow + ps] = (uint16_t)(lv[c][t >> 1] >> (16 * (t & 1)));
+        lb[row * kWgRow + ps] = (uint16_t)(rv[c][t >> 1] >> (16 * (t & 1)));
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    bf16x8 af[2], bf[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      af[u] = *(const bf16x8 *)(la + (32 * u + l32) * kWgRow + 8 * half);
+      bf[u] = *(const bf16x8 *)(lb + (32 * u + l32) * kWgRow + 8 * half);
+    }
+    if (bias_tile) {
+      const bf16x8 r0 = *(const bf16x8 *)(la + lane * kWgRow);
+      const bf16x8 r1 = *(const bf16x8 *)(la + lane * kWgRow + 8);
+      float t0 = 0.f;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) t0 += (float)r0[t];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) t0 += (float)r1[t];
+      lsum += t0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // the next step overwrites this wavefront's LDS rows
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+        acc[u][v] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[u], bf[v], acc[u][v], 0, 0, 0);
+    x += 16;
+    while (x >= a.W) {
+      x -= a.W;
+      ++yg;
+      if (++yi == a.H) yi = 0;
+    }
   }
-  if (!jok) return;
+  if (bias_tile && i0 + lane < a.I) a.bias_out[(long long)s * a.I + i0 + lane] = lsum;
   float *out = a.out + (long long)s * a.I * a.J;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int ii = ti * 32 + acc_row(r, half);
-    if (ii < a.I) out[(long long)ii * a.J + j] = acc[r];
+  for (int v = 0; v < 2; ++v) {
+    const int j = j0 + 32 * v + l32;
+    if (j >= a.J) continue;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int ii = i0 + 32 * u + acc_row(r, half);
+        if (ii < a.I) out[(long long)ii * a.J + j] = acc[u][v][r];
+      }
   }
 }
 
-// out[e] = sum_s part[s * n + e], s in order (deterministic).
-__global__ void sum_splits_kernel(const float *__restrict__ part, float *out, long long n,
-                                  int nsplit) {
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  float v = 0.f;
-  for (int s = 0; s < nsplit; ++s) v += part[(long long)s * n + e];
-  out[e] = v;
+// out[e] = sum over s < nsplit of part[s * n + e]: `so` split lanes per
+// output (a power of two dividing 256), each summing every so-th split, then
+// a tree over the lanes -- a fixed order for a given (n, nsplit).
+__global__ void __launch_bounds__(256) reduce_splits_kernel(const float *__restrict__ part,
+                                                            float *out, long long n,
+                                                            int nsplit, int so) {
+  __shared__ float red[256];
+  const int eo = 256 / so, el = threadIdx.x % eo, sl = threadIdx.x / eo;
+  const long long e = (long long)blockIdx.x * eo + el;
+  float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f;
+  if (e < n) {
+    int sp = sl;
+    for (; sp + 3 * so < nsplit; sp += 4 * so) {
+      v0 += part[(long long)sp * n + e];
+      v1 += part[(long long)(sp + so) * n + e];
+      v2 += part[(long long)(sp + 2 * so) * n + e];
+      v3 += part[(long long)(sp + 3 * so) * n + e];
+    }
+    for (; sp < nsplit; sp += so) v0 += part[(long long)sp * n + e];
+  }
+  red[threadIdx.x] = (v0 + v1) + (v2 + v3);
+  __syncthreads();
+  for (int w = so / 2; w >= 1; w /= 2) {
+    if (sl < w) red[sl * eo + el] += red[(sl + w) * eo + el];
+    __syncthreads();
+  }
+  if (sl == 0 && e < n) out[e] = red[el];
 }
 
 // part[s][c] = sum of src[p * stride + off + c] over the pixels of split s;
@@ -212,10 +342,16 @@ __global__ void __launch_bounds__(256) channel_sum_kernel(const uint16_t *__rest
   const int c = blockIdx.x * 64 + (threadIdx.x & 63), r = threadIdx.x >> 6;
   const long long p_begin = (long long)blockIdx.y * chunk;
   const long long p_end = p_begin + chunk < P ? p_begin + chunk : P;
-  float v = 0.f;
-  if (c < C)
-    for (long long p = p_begin + r; p < p_end; p += 4) v += from_bf16(src[p * stride + off + c]);
-  red[r][threadIdx.x & 63] = v;
+  float v0 = 0.f, v1 = 0.f;
+  if (c < C) {
+    long long p = p_begin + r;
+    for (; p + 4 < p_end; p += 8) {
+      v0 += from_bf16(src[p * stride + off + c]);
+      v1 += from_bf16(src[(p + 4) * stride + off + c]);
+    }
+    if (p < p_end) v0 += from_bf16(src[p * stride + off + c]);
+  }
+  red[r][threadIdx.x & 63] = v0 + v1;
   __syncthreads();
   if (r == 0 && c < C)
     part[(long long)blockIdx.y * C + c] =
@@ -284,42 +420,61 @@ hipError_t launch_conv_gemm(const ConvGemm &a, hipStream_t s) {
       (a.mode == kConv && a.ks != 1 && a.ks != 3) || (a.mode != kConv && a.ks != 1) ||
       a.mode < kConv || a.mode > kGather2x2 || (a.pre16 && a.mode != kConv))
     return hipErrorInvalidValue;
+  if (((uintptr_t)a.in | (uintptr_t)a.w) & 15) return hipErrorInvalidValue;  // 16-byte loads
   const long long M = (long long)a.rows * a.W, N = a.mode == kScatter ? 4LL * a.cout : a.cout;
-  const long long tiles = ((M + 31) / 32) * ((N + 31) / 32);
-  hipLaunchKernelGGL(conv_mfma_kernel, dim3(blocks_of(tiles, 4)), dim3(256), 0, s, a);
+  if (N > 32) {  // 64 x 64 per wavefront
+    const long long tiles = ((M + 63) / 64) * ((N + 63) / 64);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(conv_mfma_kernel<2, 2>), dim3(blocks_of(tiles, 4)),
+                       dim3(256), 0, s, a);
+  } else {  // 128 x 32
+    const long long tiles = ((M + 127) / 128) * ((N + 31) / 32);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(conv_mfma_kernel<4, 1>), dim3(blocks_of(tiles, 4)),
+                       dim3(256), 0, s, a);
+  }
   return hipGetLastError();
+}
+
+void launch_reduce_splits(const float *part, float *out, long long n, int nsplit,
+                          hipStream_t s) {
+  int so = 1;
+  while (so < 256 && (long long)so * 8 < nsplit) so *= 2;
+  hipLaunchKernelGGL(reduce_splits_kernel, dim3(blocks_of(n, 256 / so)), dim3(256), 0, s,
+                     part, out, n, nsplit, so);
 }
 
 long long wgrad_plan(Wgrad &a, long long workspace_floats) {
   const long long P = (long long)a.rows * a.W;
-  const long long tiles = (long long)((a.I + 31) / 32) * ((a.J + 31) / 32);
+  const long long tiles = (long long)((a.I + 63) / 64) * ((a.J + 63) / 64);
   long long ns = (2048 + tiles - 1) / tiles;
-  const long long max_ns = (P + 255) / 256;
+  const long long max_ns = (P + 511) / 512;
   if (ns > max_ns) ns = max_ns;
-  const long long fit = workspace_floats / ((long long)a.I * a.J);
+  const long long fit = workspace_floats / ((long long)a.I * a.J + a.I);
   if (ns > fit) ns = fit;
   if (ns < 1) ns = 1;
   long long chunk = (P + ns - 1) / ns;
-  chunk = (chunk + 7) / 8 * 8;
+  chunk = (chunk + 15) / 16 * 16;
   a.nsplit = (int)((P + chunk - 1) / chunk);
   a.chunk = chunk;
   return a.nsplit;
 }
 
-hipError_t launch_wgrad(Wgrad a, float *dw, float *workspace, long long workspace_floats,
-                        hipStream_t s) {
+hipError_t launch_wgrad(Wgrad a, float *dw, float *db, float *workspace,
+                        long long workspace_floats, hipStream_t s) {
   if (a.I <= 0 || a.J <= 0 || a.rows <= 0 || a.H <= 0 || a.rows % a.H || a.W <= 0 ||
-      (a.ks != 1 && a.ks != 3) || a.rc <= 0 || a.J != a.ks * a.ks * a.rc ||
-      (a.l_s2d && (a.lc <= 0 || a.I != 4 * a.lc)))
+      (a.ks != 1 && a.ks != 3) || a.rc <= 0 || a.rc % 8 || a.J != a.ks * a.ks * a.rc ||
+      a.l_stride % 8 || a.r_stride % 8 || (((uintptr_t)a.l | (uintptr_t)a.r) & 15) ||
+      (a.l_s2d && (a.lc <= 0 || a.lc % 8 || a.I != 4 * a.lc)))
     return hipErrorInvalidValue;
   wgrad_plan(a, workspace ? workspace_floats : 0);
-  a.out = a.nsplit > 1 ? workspace : dw;
-  const long long tiles = (long long)((a.I + 31) / 32) * ((a.J + 31) / 32) * a.nsplit;
+  const long long n = (long long)a.I * a.J;
+  const bool split = a.nsplit > 1;
+  a.out = split ? workspace : dw;
+  a.bias_out = db ? (split ? workspace + (long long)a.nsplit * n : db) : nullptr;
+  const long long tiles = (long long)((a.I + 63) / 64) * ((a.J + 63) / 64) * a.nsplit;
   hipLaunchKernelGGL(wgrad_mfma_kernel, dim3(blocks_of(tiles, 4)), dim3(256), 0, s, a);
-  if (a.nsplit > 1) {
-    const long long n = (long long)a.I * a.J;
-    hipLaunchKernelGGL(sum_splits_kernel, dim3(blocks_of(n, 256)), dim3(256), 0, s,
-                       workspace, dw, n, a.nsplit);
+  if (split) {
+    launch_reduce_splits(workspace, dw, n, a.nsplit, s);
+    if (db) launch_reduce_splits(a.bias_out, db, a.I, a.nsplit, s);
   }
   return hipGetLastError();
 }
@@ -328,16 +483,15 @@ hipError_t launch_channel_sum(const uint16_t *src, int stride, int off, int c, l
                               float *out, float *workspace, long long workspace_floats,
                               hipStream_t s) {
   if (c <= 0 || P <= 0 || off < 0 || off + c > stride) return hipErrorInvalidValue;
-  long long ns = (P + 4095) / 4096;
-  if (ns > 256) ns = 256;
-  if (!workspace || ns > workspace_floats / c) ns = 1;
+  long long ns = (P + 511) / 512;
+  if (ns > 4096) ns = 4096;
+  if (!workspace || ns > workspace_floats / c) ns = workspace ? workspace_floats / c : 1;
+  if (ns < 1) ns = 1;
   const long long chunk = (P + ns - 1) / ns;
   ns = (P + chunk - 1) / chunk;
   hipLaunchKernelGGL(channel_sum_kernel, dim3(blocks_of(c, 64), (unsigned)ns), dim3(256), 0,
                      s, src, stride, off, c, P, chunk, ns > 1 ? workspace : out);
-  if (ns > 1)
-    hipLaunchKernelGGL(sum_splits_kernel, dim3(blocks_of(c, 256)), dim3(256), 0, s,
-                       workspace, out, (long long)c, (int)ns);
+  if (ns > 1) launch_reduce_splits(workspace, out, (long long)c, (int)ns, s);
   return hipGetLastError();
 }
 

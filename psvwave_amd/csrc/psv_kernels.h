@@ -247,17 +247,19 @@ struct ConvGemm {
 };
 hipError_t launch_conv_gemm(const ConvGemm &a, hipStream_t s);
 // Weight gradient out[i][j] = sum_p L[p][i] R[p][j] over the rows x W
-// pixels p (see psv_conv.hip); fp32 result in dw, split partial sums in
-// workspace (fewer splits when it is small or null).
+// pixels p (see psv_conv.hip); fp32 result in dw and, when db is given,
+// db[i] = sum_p L[p][i]; split partial sums in workspace (fewer splits when
+// it is small or null).
 struct Wgrad {
   const uint16_t *l;
   const uint16_t *r;
   float *out;
+  float *bias_out;  // per-split sums of L over the pixels [nsplit][I], or null
   long long chunk;
   int l_stride, l_s2d, lc, r_stride, rc, ks, I, J, rows, H, W, nsplit;
 };
-hipError_t launch_wgrad(Wgrad a, float *dw, float *workspace, long long workspace_floats,
-                        hipStream_t s);
+hipError_t launch_wgrad(Wgrad a, float *dw, float *db, float *workspace,
+                        long long workspace_floats, hipStream_t s);
 hipError_t launch_channel_sum(const uint16_t *src, int stride, int off, int c, long long P,
                               float *out, float *workspace, long long workspace_floats,
                               hipStream_t s);

@@ -260,7 +260,7 @@ def learned_direction(net, g, fields, device="cuda"):
 
 
 def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
-        target=None, log=None):
+        target=None, log=None, lcap=1.0, expand=False):
     """Steepest-descent FWI from `start` against model's observed data.
 
     Each iteration computes the gradient and tries, in order, the
@@ -269,8 +269,11 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
     plain direction (-g, each parameter block scaled so a step
     changes it by at most step0 times its largest value, times an adaptive
     step factor), halving each until the plain-forward misfit decreases (the
-    plain direction is the fallback).  Returns the history: list of
-    (elapsed seconds, misfit, used fallback)."""
+    plain direction is the fallback).  The learned scale is capped at lcap
+    times the network's own; with expand, an accepted first trial is
+    followed by one at twice the step, kept when it lowers the misfit
+    further.  Returns the history: list of (elapsed seconds, misfit, used
+    fallback)."""
     fields = [np.array(f, dtype=np.float64) for f in start]
     model.set_parameter_fields(*fields)
     t0 = time.perf_counter()
@@ -301,15 +304,20 @@ def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
             else:
                 scale = step0
                 a = scale * factor
-            for _ in range(max_halvings):
+            for k in range(max_halvings):
                 trial = [f + a * pp for f, pp in zip(fields, p)]
                 ft = misfit_at(model, trial)
                 if ft < f0:
+                    if expand and k == 0:
+                        trial2 = [f + 2 * a * pp for f, pp in zip(fields, p)]
+                        ft2 = misfit_at(model, trial2)
+                        if ft2 < ft:
+                            trial, ft, a = trial2, ft2, 2 * a
                     fields = trial
                     if name == "plain":
                         factor = min(2.0 * a / scale, 4.0)
                     else:
-                        lfactor = min(1.5 * a, 1.0)
+                        lfactor = min(1.5 * a, lcap)
                     accepted = True
                     break
                 a *= 0.5
@@ -511,18 +519,14 @@ class HipUNetTrainer:
         return out
 
     def _wgrad(self, dy, dy_stride, gather, dy_ch, x, x_stride, x_ch, ks, bhw, dw):
+        """Weight gradient into dw; returns the bias gradient."""
         b, h, w = bhw
+        db = self._empty(4 * dy_ch if gather else dy_ch, f32=True)
         self.check(self.lib.psv_conv_wgrad_bf16(
             dy.data_ptr(), dy_stride, int(gather), dy_ch, x.data_ptr(), x_stride, x_ch, ks,
-            b, h, w, dw.data_ptr(), self.ws.data_ptr(), self.ws.numel(), self._stream()))
-        return dw
-
-    def _chsum(self, src, stride, off, ch, pixels):
-        out = self._empty(ch, f32=True)
-        self.check(self.lib.psv_channel_sum_bf16(
-            src.data_ptr(), stride, off, ch, pixels, out.data_ptr(), self.ws.data_ptr(),
-            self.ws.numel(), self._stream()))
-        return out
+            b, h, w, dw.data_ptr(), db.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
+            self._stream()))
+        return db.view(4, dy_ch).sum(0) if gather else db
 
     # ---- network --------------------------------------------------------
     def _block_forward(self, x, cin_stride, cin, L, bhw):
@@ -578,16 +582,15 @@ class HipUNetTrainer:
         input (fp32 or bf16), or None."""
         c1, c2 = L
         b, h, w = s["bhw"]
-        P = b * h * w
         dw2 = self._empty(c2["co"], 9 * c2["c8"], f32=True)
-        self._wgrad(dz2, c2["co"], 0, c2["co"], s["a1"], c1["co"], c2["c8"], 3, s["bhw"], dw2)
-        grads[c2["mod"]] = (dw2, self._chsum(dz2, c2["co"], 0, c2["co"], P))
+        grads[c2["mod"]] = (dw2, self._wgrad(dz2, c2["co"], 0, c2["co"], s["a1"], c1["co"],
+                                             c2["c8"], 3, s["bhw"], dw2))
         dz1 = self._empty(b, h, w, c1["co"])
         self._conv(dz2, c2["co"], c2["o8"], c2["wd"], None, dz1, c1["co"], 0, c2["ci"],
                    s["bhw"], dz=s["z1"], dz_stride=c1["co"])
         dw1 = self._empty(c1["co"], 9 * c1["c8"], f32=True)
-        self._wgrad(dz1, c1["co"], 0, c1["co"], s["x"], s["xs"], c1["c8"], 3, s["bhw"], dw1)
-        grads[c1["mod"]] = (dw1, self._chsum(dz1, c1["co"], 0, c1["co"], P))
+        grads[c1["mod"]] = (dw1, self._wgrad(dz1, c1["co"], 0, c1["co"], s["x"], s["xs"],
+                                             c1["c8"], 3, s["bhw"], dw1))
         if not need_dx:
             return None
         dx = self._empty(b, h, w, c1["ci"], f32=dx_f32)
@@ -606,8 +609,8 @@ class HipUNetTrainer:
         dyo[..., :o["co"]] = dy
         last = saved["dec"][2]
         dwo = self._empty(o["co"], o["c8"], f32=True)
-        self._wgrad(dyo, o["o8"], 0, o["co"], last["a2"], o["ci"], o["c8"], 1, (b, h, w), dwo)
-        grads[o["mod"]] = (dwo, self._chsum(dyo, o["o8"], 0, o["co"], b * h * w))
+        grads[o["mod"]] = (dwo, self._wgrad(dyo, o["o8"], 0, o["co"], last["a2"], o["ci"],
+                                            o["c8"], 1, (b, h, w), dwo))
         dz2 = self._empty(b, h, w, o["ci"])
         self._conv(dyo, o["o8"], o["o8"], o["wd"], None, dz2, o["ci"], 0, o["ci"], (b, h, w),
                    ks=1, dz=last["z2"], dz_stride=o["ci"])
@@ -620,8 +623,8 @@ class HipUNetTrainer:
             dskips[2 - lvl] = (dcat, ccat, cu)
             lo = (bb, hh // 2, ww // 2)
             dwu = self._empty(4 * cu, up["ci"], f32=True)
-            self._wgrad(dcat, ccat, 1, cu, saved["up_in"][lvl], up["ci"], up["ci"], 1, lo, dwu)
-            grads[up["mod"]] = (dwu, self._chsum(dcat, ccat, 0, cu, bb * hh * ww))
+            grads[up["mod"]] = (dwu, self._wgrad(dcat, ccat, 1, cu, saved["up_in"][lvl],
+                                                 up["ci"], up["ci"], 1, lo, dwu))
             prev = saved["mid"] if lvl == 0 else saved["dec"][lvl - 1]
             dz2 = self._empty(bb, hh // 2, ww // 2, up["ci"])
             self._conv(dcat, ccat, cu, up["wd"], None, dz2, up["ci"], 0, up["ci"], lo, ks=1,

@@ -174,14 +174,16 @@ def test_training_convolutions_match_autograd(batch, h, w, cin, cout, ks):
     ws = torch.empty(1 << 20, device="cuda")
     for workspace in (ws, None):
         dw = torch.full((cout, ks * ks * cin), float("nan"), device="cuda")
+        dbw = torch.full((cout,), float("nan"), device="cuda")
         _lib.check(lib.psv_conv_wgrad_bf16(dyd.data_ptr(), o8, 0, cout, xd.data_ptr(), cin, cin,
-                                           ks, batch, h, w, dw.data_ptr(),
+                                           ks, batch, h, w, dw.data_ptr(), dbw.data_ptr(),
                                            None if workspace is None else workspace.data_ptr(),
                                            0 if workspace is None else workspace.numel(),
                                            stream))
         got = dw.cpu().view(cout, ks, ks, cin).permute(0, 3, 1, 2)
         assert torch.allclose(got.double(), wt.grad, rtol=1e-4, atol=1e-4), \
             (got.double() - wt.grad).abs().max()
+        assert torch.allclose(dbw.cpu().double(), b.grad, rtol=1e-5, atol=1e-4)
     db = torch.empty(cout, device="cuda")
     _lib.check(lib.psv_channel_sum_bf16(dyd.data_ptr(), o8, 0, cout, batch * h * w,
                                         db.data_ptr(), ws.data_ptr(), ws.numel(), stream))
@@ -213,12 +215,15 @@ def test_transposed_convolution_gradients():
     assert torch.allclose(dx.cpu(), ref, rtol=1e-4, atol=1e-4)
     xd = _nhwc(x.detach()).to("cuda", bf)
     dw = torch.empty(4 * cout, cin, device="cuda")
+    db = torch.empty(4 * cout, device="cuda")
     ws = torch.empty(1 << 20, device="cuda")
     _lib.check(lib.psv_conv_wgrad_bf16(dyd.data_ptr(), cout + extra, 1, cout, xd.data_ptr(), cin,
-                                       cin, 1, batch, h, w, dw.data_ptr(), ws.data_ptr(),
-                                       ws.numel(), stream))
+                                       cin, 1, batch, h, w, dw.data_ptr(), db.data_ptr(),
+                                       ws.data_ptr(), ws.numel(), stream))
     got = dw.cpu().view(2, 2, cout, cin).permute(3, 2, 0, 1).double()
     assert torch.allclose(got, wt.grad, rtol=1e-4, atol=1e-4)
+    ref_db = dy.sum(dim=(0, 2, 3))
+    assert torch.allclose(db.cpu().double().view(4, cout).sum(0), ref_db, rtol=1e-5, atol=1e-4)
 
 
 def test_pool_backward_routes_like_torch():
