@@ -83,11 +83,15 @@ def _args():
                    help="first run a plain copy kernel over BYTES (the "
                         "reference point for byte counters) and report it")
     p.add_argument("--only-calibrate", action="store_true")
-    p.add_argument("--workload", choices=["single", "mc", "c4", "c5"], default="single",
+    p.add_argument("--workload", choices=["single", "mc", "c4", "c5", "unet"],
+                   default="single",
                    help="single: the layered-faulted model (default); mc: the "
                         "512-model Monte-Carlo forward batch; c4: gradient "
                         "through torch.autograd with checkpointing vs full "
-                        "snapshot storage, checked by finite differences")
+                        "snapshot storage, checked by finite differences; c5: "
+                        "FWI time-to-misfit with the learned preconditioner; "
+                        "unet: the preconditioner network's training step and "
+                        "inference on the library's MFMA kernels vs torch")
     p.add_argument("--c4-directions", type=int, default=20)
     p.add_argument("--c5-pairs", type=int, default=2048)
     p.add_argument("--c5-epochs", type=int, default=20)
@@ -358,6 +362,102 @@ def learned_fwi(a):
     }
 
 
+def _unet_flops(h, w, batch, base=32):
+    """Multiply-add FLOPs of one forward pass of the U-Net on batch images
+    of h x w (padded) pixels: 2 * pixels * K * N per convolution."""
+    c = [base, 2 * base, 4 * base, 8 * base]
+    f, hh, ww = 0, h, w
+    ins = [3, c[0], c[1]]
+    for lvl in range(3):
+        f += 2 * hh * ww * 9 * (ins[lvl] * c[lvl] + c[lvl] * c[lvl])
+        hh, ww = hh // 2, ww // 2
+    f += 2 * hh * ww * 9 * (c[2] * c[3] + c[3] * c[3])
+    for lvl in range(3):
+        cu = c[2 - lvl]
+        f += 2 * hh * ww * 4 * c[3 - lvl] * cu  # 2x2 stride-2 transposed convolution
+        hh, ww = 2 * hh, 2 * ww
+        f += 2 * hh * ww * 9 * (2 * cu * cu + cu * cu)
+    f += 2 * hh * ww * c[0] * 3
+    return batch * f
+
+
+def unet_kernels(a):
+    """The preconditioner network on the library's bfloat16 MFMA kernels:
+    one AdamW training step on a batch of 16 gradients of the 316 x 188 C5
+    grid (forward, input gradients, weight gradients), and the inference of
+    one gradient, against torch's bfloat16 autocast (MIOpen) on the same
+    data.  FLOPs count 2 * pixels * K * N per convolution on the padded
+    320 x 192 images; a training step is 3 x the forward (forward, input
+    gradient, weight gradient)."""
+    import copy
+    import torch
+    from psvwave_amd import precond
+    dev = torch.device("cuda", 0)
+    torch.manual_seed(0)
+    net = precond.make_unet().to(dev)
+    net_t = copy.deepcopy(net)
+    tr = precond.HipUNetTrainer(net)
+    rng = np.random.default_rng(0)
+    nx, nz, batch = 316, 188, 16
+    x = torch.tensor(precond.normalise_gradient(
+        rng.standard_normal((batch, 3, nx, nz)).astype(np.float32)), device=dev)
+    y = torch.tensor(0.02 * rng.standard_normal((batch, 3, nx, nz)), dtype=torch.float32,
+                     device=dev)
+    mask = torch.tensor(precond.kernel_mask((nx, nz)), dtype=torch.float32, device=dev)
+    opt = torch.optim.AdamW(net_t.parameters(), lr=2e-3, weight_decay=1e-4)
+
+    def torch_step():
+        with torch.autocast(device_type="cuda", dtype=torch.bfloat16):
+            pred = net_t(x)
+        loss = (((pred.float() - y) * mask) ** 2).mean() / (mask.mean() * 1e-3)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+
+    unet = precond.HipUNet(net.eval())
+    x1 = x[0].cpu().numpy()
+    x1d = x[:1]
+
+    def torch_infer():
+        with torch.no_grad(), torch.autocast(device_type="cuda", dtype=torch.bfloat16):
+            net_t(x1d)
+
+    def timed(f, n):
+        for _ in range(a.warmup + 1):
+            f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    n = max(a.steps, 10)
+    res = {"hip_train_ms": timed(lambda: tr.step(x, y, mask), n),
+           "torch_train_ms": timed(torch_step, n),
+           "hip_infer_ms": timed(lambda: unet(x1), n),
+           "torch_infer_ms": timed(torch_infer, n)}
+    _progress(" ".join(f"{k} {v:.3f}" for k, v in res.items()))
+    flops = 3 * _unet_flops(320, 192, batch)
+    tf = flops / (res["hip_train_ms"] * 1e-3) / 1e12
+    return {
+        "metric": "U-Net training step time (bf16 MFMA kernels)",
+        "value": round(res["hip_train_ms"], 3), "unit": "ms per training step",
+        "higher_is_better": False, "n_gpus": 1, "steps": n, "warmup": a.warmup,
+        "dtype": "bf16 operands, fp32 accumulation", "data": "synthetic (random gradients)",
+        "config": {"workload": "C5 preconditioner network", "batch": batch,
+                   "grid": f"{nx}x{nz} padded to 320x192", "parameters": 1.9e6},
+        "torch_autocast_train_ms": round(res["torch_train_ms"], 3),
+        "speedup_vs_torch_train": round(res["torch_train_ms"] / res["hip_train_ms"], 3),
+        "hip_infer_ms": round(res["hip_infer_ms"], 3),
+        "torch_autocast_infer_ms": round(res["torch_infer_ms"], 3),
+        "roofline": {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": round(tf / 2500.0, 4),
+                     "note": "whole training step (all kernels, host work "
+                             "included) against dense bf16 MFMA peak"},
+    }
+
+
 def main():
     a = _args()
     if a.gpus != 1:
@@ -378,6 +478,9 @@ def main():
         return
     if a.workload == "c4":
         print(json.dumps(gradient_check(a)), flush=True)
+        return
+    if a.workload == "unet":
+        print(json.dumps(unet_kernels(a)), flush=True)
         return
     if a.workload == "c5":
         print(json.dumps(learned_fwi(a)), flush=True)

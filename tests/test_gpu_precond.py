@@ -1,7 +1,8 @@
 """The learned-preconditioner pipeline at a small scale: training pairs from
-the device solver, a few optimisation steps of the U-Net, and FWI with and
-without it (each accepted step lowers the misfit; the plain direction is
-the fallback)."""
+the device solver, a few optimisation steps of the U-Net on the library's
+kernels, and FWI without it and with it (inference on the library's kernels
+and, for comparison, torch's): each accepted step lowers the misfit, and
+the plain direction is the fallback."""
 import numpy as np
 import pytest
 
@@ -22,16 +23,18 @@ def test_precond_pipeline_small():
     assert np.all(g[:, :, mask == 0] == 0)
     torch.manual_seed(0)
     net = precond.make_unet(base=8)
-    hist = precond.train(net, g, p, epochs=3, batch=3)
+    hist = precond.train_hip(net, g, p, epochs=3, batch=3)  # the library's kernels
     assert np.isfinite(hist).all()
     net.eval()
+    unet = precond.HipUNet(net)
     bg, true = precond.c5_models(np.random.default_rng(3))
     m = psvwave_amd.fdModel(cfg)
     m.set_parameter_fields(*true)
     for s in range(m.n_shots):
         m.forward_simulate(s, store_fields=False)
     m.set_observed_data(*m.get_synthetic_data())
-    for direction in (None, lambda gg, f: precond.learned_direction(net, gg, f)):
+    for direction in (None, lambda gg, f: precond.learned_direction_hip(unet, gg, f),
+                      lambda gg, f: precond.learned_direction(net, gg, f)):
         h, fields = precond.fwi(m, bg, iterations=3, direction=direction)
         misfits = [x[1] for x in h]
         assert all(b <= a for a, b in zip(misfits, misfits[1:]))
