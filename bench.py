@@ -68,6 +68,12 @@ def _args():
                         "tiles (9 is the library default); 14 lean two "
                         "steps per launch (64x8)")
     p.add_argument("--no-xcd-remap", action="store_true")
+    p.add_argument("--sequential", action="store_true",
+                   help="drive the shots call by call (forward_simulate per "
+                        "shot, misfit, adjoint sources, adjoint_simulate per "
+                        "shot) instead of run_model, which overlaps the "
+                        "forward of shot s + 1 with the adjoint of shot s "
+                        "on two HIP streams (bitwise the same results)")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
     p.add_argument("--timing-stride", type=int, default=-1,
@@ -505,6 +511,11 @@ def main():
     model.set_parameter_fields(*start)
 
     def step():
+        if not a.sequential:
+            # src/fdModel.cpp:1291-1308 (forward all shots, misfit, adjoint
+            # sources, reset, adjoint all shots, map kernels)
+            model.run_model(False, True)
+            return
         for s in range(model.n_shots):
             model.forward_simulate(s)
         model.calculate_l2_misfit()
@@ -572,6 +583,11 @@ def main():
             "step_kernel": kernel_name,
             "xcd_remap": not a.no_xcd_remap,
             "hip_graph": a.graph,
+            "driver": ("call by call, one stream" if a.sequential or a.graph or
+                       a.timing_stride > 0 else
+                       "run_model: forward of shot s+1 on a second HIP stream "
+                       "beside the adjoint of shot s (bitwise the sequential "
+                       "results)"),
         },
         "roofline": {
             "bound": "hbm",
@@ -586,9 +602,13 @@ def main():
             "timed_launches": timed,
             "launches": launches,
             "launches_per_time_step": launches_per_kernel,
-            "timing": ("HIP events around each time loop, per time step, gaps "
-                       "included" if a.timing_stride < 0 else
-                       f"kernel start/end of every {a.timing_stride}th launch"),
+            "timing": (f"kernel start/end of every {a.timing_stride}th launch"
+                       if a.timing_stride > 0 else
+                       "HIP events around each time loop, per time step, gaps "
+                       "included" if a.sequential or a.graph else
+                       "HIP events around each run_model (both streams), per "
+                       "time step of every shot: wall time per launch while "
+                       "two launches overlap"),
         },
     }
     if copy:

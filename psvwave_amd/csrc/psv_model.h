@@ -157,7 +157,16 @@ class Model {
   int enqueue_range(StepArgs &a, double *(*st)[5], int parity, bool forward,
                     int k0, int k1, bool snaps, const SlotFn &slot,
                     std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
-                    bool dump);
+                    bool dump, hipStream_t s = nullptr);
+  StepArgs forward_args(int i_shot) const;
+  StepArgs adjoint_args(int i_shot) const;
+  // run_model with two lanes: forward time loops on lane_stream_ (state in
+  // lane_state_), adjoint time loops in shot order on stream_; see
+  // psv_model.hip.
+  void run_model_two_lanes(bool simulate_adjoint);
+  hipStream_t lane_stream_ = nullptr;
+  double *lane_state_[2][5] = {};
+  std::vector<hipEvent_t> lane_events_;
   void enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
                     std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
                     bool dump);

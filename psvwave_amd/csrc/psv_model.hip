@@ -88,8 +88,11 @@ Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
 
 Model::~Model() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (lane_stream_) (void)hipStreamSynchronize(lane_stream_);
   clear_graphs();
   for (auto e : events_) (void)hipEventDestroy(e);
+  for (auto e : lane_events_) (void)hipEventDestroy(e);
+  if (lane_stream_) (void)hipStreamDestroy(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -479,7 +482,8 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
                          bool forward, int k0, int k1, bool snaps,
                          const SlotFn &slot,
                          std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
-                         bool dump) {
+                         bool dump, hipStream_t s) {
+  hipStream_t ls = s ? s : stream_;
   const bool inplace = step_is_inplace(variant_);
   const bool two = step_is_two_step(variant_) && !dump;
   const int single = (two || dump) ? step_single_variant(variant_) : variant_;
@@ -522,24 +526,24 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
       pairs->emplace_back(e0, e1);
     }
     if (pair)
-      hip_check(launch_two_steps(variant_, a, stream_, e0, e1), "two-step kernel");
+      hip_check(launch_two_steps(variant_, a, ls, e0, e1), "two-step kernel");
     else
-      hip_check(launch_step(single, a, stream_, e0, e1), "step kernel");
+      hip_check(launch_step(single, a, ls, e0, e1), "step kernel");
     launches_ += covered;
     if (timed) timed_ += covered;
     if (!inplace) parity = nxt;
     if (dump && it % 10 == 0) {
       // src/fdModel.cpp:753-776: state after step it, default stream format
       std::vector<double> hx(ngrid_), hz(ngrid_);
-      double *s = staging(ngrid_);
-      hip_check(launch_unpad(st[nxt][0], s, d_.nx, d_.nz, pitch_, origin_,
-                             stream_),
+      double *sb = staging(ngrid_);
+      hip_check(launch_unpad(st[nxt][0], sb, d_.nx, d_.nz, pitch_, origin_,
+                             ls),
                 "unpad");
-      to_host_layout(hx.data(), s);
-      hip_check(launch_unpad(st[nxt][1], s, d_.nx, d_.nz, pitch_, origin_,
-                             stream_),
+      to_host_layout(hx.data(), sb);
+      hip_check(launch_unpad(st[nxt][1], sb, d_.nx, d_.nz, pitch_, origin_,
+                             ls),
                 "unpad");
-      to_host_layout(hz.data(), s);
+      to_host_layout(hz.data(), sb);
       std::ofstream fx("snapshots/vx" + zero_pad(it, 5) + ".txt");
       std::ofstream fz("snapshots/vz" + zero_pad(it, 5) + ".txt");
       for (int ix = 0; ix < d_.nx; ++ix) {
@@ -798,15 +802,7 @@ void Model::forward(int i_shot, bool store_fields, bool verbose,
     throw ValueError("Snapshot interval and size of accumulator don't match!"
                      " (nt must be a multiple of snapshot_interval)");
   if (store_fields) ensure_snapshots();
-  StepArgs a = base_args();
-  a.adjoint = 0;
-  a.dts = cfg_.dt;
-  a.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
-  a.entries = fwd_entries_;
-  a.halo_off = fwd_halo_off_ + (size_t)i_shot * (ntiles_ + 1);
-  a.halo_entries = fwd_halo_;
-  a.rtf_x = syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
-  a.rtf_z = syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  StepArgs a = forward_args(i_shot);
   auto t0 = std::chrono::steady_clock::now();
   run_steps(a, i_shot, true, store_fields, output_wavefields);
   if (verbose) {
@@ -825,15 +821,7 @@ void Model::adjoint(int i_shot, bool verbose) {
   if (!snap_[0])
     throw ValueError("adjoint_simulate needs stored forward wavefields; run "
                      "forward_simulate with store_fields=True first");
-  StepArgs a = base_args();
-  a.adjoint = 1;
-  a.dts = -cfg_.dt;
-  a.tile_off = adj_off_;
-  a.entries = adj_entries_;
-  a.halo_off = adj_halo_off_;
-  a.halo_entries = adj_halo_;
-  a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
-  a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  StepArgs a = adjoint_args(i_shot);
   auto t0 = std::chrono::steady_clock::now();
   if (ckpt_every_ > 1 && timing_stride_ < 0) {
     timed_loop([&] { enqueue_checkpointed_adjoint(a, i_shot, nullptr); });
@@ -888,8 +876,44 @@ void Model::map_kernels_to_velocity() {
   sync();
 }
 
-// src/fdModel.cpp:1291-1308
+// Arguments of the time loops of one shot (src/fdModel.cpp:526-551, 789-815).
+StepArgs Model::forward_args(int i_shot) const {
+  StepArgs a = base_args();
+  a.adjoint = 0;
+  a.dts = cfg_.dt;
+  a.tile_off = fwd_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.entries = fwd_entries_;
+  a.halo_off = fwd_halo_off_ + (size_t)i_shot * (ntiles_ + 1);
+  a.halo_entries = fwd_halo_;
+  a.rtf_x = syn_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.rtf_z = syn_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  return a;
+}
+
+StepArgs Model::adjoint_args(int i_shot) const {
+  StepArgs a = base_args();
+  a.adjoint = 1;
+  a.dts = -cfg_.dt;
+  a.tile_off = adj_off_;
+  a.entries = adj_entries_;
+  a.halo_off = adj_halo_off_;
+  a.halo_entries = adj_halo_;
+  a.adj_x = adj_[0] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  a.adj_z = adj_[1] + (size_t)i_shot * cfg_.nr * cfg_.nt;
+  return a;
+}
+
+// src/fdModel.cpp:1291-1308: forward all shots (storing snapshots), misfit,
+// and optionally adjoint sources, reset, adjoint all shots, map kernels.
 void Model::run_model(bool verbose, bool simulate_adjoint) {
+  if (cfg_.nt % cfg_.snapshot_interval != 0)
+    throw ValueError("Snapshot interval and size of accumulator don't match!"
+                     " (nt must be a multiple of snapshot_interval)");
+  if (!verbose && !graph_mode_ && ckpt_every_ == 1 && timing_stride_ <= 0 &&
+      cfg_.n_shots > 1) {
+    run_model_two_lanes(simulate_adjoint);
+    return;
+  }
   for (int s = 0; s < cfg_.n_shots; ++s) forward(s, true, verbose, false);
   calculate_l2_misfit();
   if (simulate_adjoint) {
@@ -897,6 +921,138 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
     reset_kernels();
     for (int s = 0; s < cfg_.n_shots; ++s) adjoint(s, verbose);
     map_kernels_to_velocity();
+  }
+}
+
+// run_model on two HIP streams.  A single time loop leaves the GPU partly
+// idle at the end of every launch (the last blocks of a launch run while
+// the next launch, which depends on them, waits); two independent loops on
+// two streams fill each other's gaps.  Lane 1 (lane_stream_, state in
+// lane_state_) runs the forward loops of all shots; lane 0 (stream_, state_)
+// runs the adjoint of shot s after the forward of shot s, in shot order, so
+// the kernel accumulation happens in the same order as one shot at a time
+// (without the adjoint, the two lanes forward alternate shots).  Every
+// kernel, input and accumulation order is that of the sequential calls, so
+// every output is bitwise the same.  The host enqueues the two lanes in
+// interleaved chunks of steps so that neither stream's queue runs dry.
+void Model::run_model_two_lanes(bool adj) {
+  const int ns = cfg_.n_shots, nt = cfg_.nt;
+  ensure_snapshots();
+  if (!lane_stream_)
+    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
+              "hipStreamCreate");
+  if (!lane_state_[0][0])
+    for (int p = 0; p < 2; ++p)
+      for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
+  while ((int)lane_events_.size() < ns + 2) {
+    hipEvent_t e;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    lane_events_.push_back(e);
+  }
+  while (events_.size() < 2) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    events_.push_back(e);
+  }
+  const bool timed = timing_stride_ < 0;
+  const bool inplace = step_is_inplace(variant_);
+  const bool two = step_is_two_step(variant_);
+  const int64_t before = launches_;
+  if (timed) hip_check(hipEventRecord(events_[0], stream_), "hipEventRecord");
+  if (adj) {
+    hip_check(launch_fill(lam_k_, ngrid_, 0.0, stream_), "reset");
+    hip_check(launch_fill(mu_k_, ngrid_, 0.0, stream_), "reset");
+    hip_check(launch_fill(rho_l_, ngrid_, 0.0, stream_), "reset");
+  }
+  // the second lane starts after everything enqueued so far on stream_
+  hip_check(hipEventRecord(lane_events_[ns], stream_), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(lane_stream_, lane_events_[ns], 0), "hipStreamWaitEvent");
+
+  struct Lane {
+    hipStream_t s;
+    double *(*st)[5];
+    std::vector<std::pair<int, bool>> jobs;  // (shot, forward)
+    size_t j = 0;
+    int k = 0, parity = 0;
+    bool started = false;
+    StepArgs a;
+  };
+  Lane lanes[2];
+  lanes[0].s = stream_;
+  lanes[0].st = state_;
+  lanes[1].s = lane_stream_;
+  lanes[1].st = lane_state_;
+  for (int sh = 0; sh < ns; ++sh) {
+    if (adj) {
+      lanes[1].jobs.emplace_back(sh, true);
+      lanes[0].jobs.emplace_back(sh, false);
+    } else {
+      lanes[1 - (sh & 1)].jobs.emplace_back(sh, true);
+    }
+  }
+  int forwards_done = 0;  // forward loops fully enqueued (lane 1 with the adjoint)
+  const int chunk = 64;
+  while (lanes[0].j < lanes[0].jobs.size() || lanes[1].j < lanes[1].jobs.size()) {
+    for (int l = 1; l >= 0; --l) {
+      Lane &ln = lanes[l];
+      if (ln.j >= ln.jobs.size()) continue;
+      const int shot = ln.jobs[ln.j].first;
+      const bool fwd = ln.jobs[ln.j].second;
+      if (!fwd && shot >= forwards_done) continue;  // its forward is not enqueued yet
+      if (!ln.started) {
+        if (fwd) {
+          ln.a = forward_args(shot);
+        } else {
+          // the adjoint source of this shot once its forward has run
+          hip_check(hipStreamWaitEvent(ln.s, lane_events_[shot], 0), "hipStreamWaitEvent");
+          const size_t off = (size_t)shot * cfg_.nr * nt;
+          hip_check(launch_adjoint_sources(syn_[0] + off, syn_[1] + off, obs_[0] + off,
+                                           obs_[1] + off, adj_[0] + off, adj_[1] + off,
+                                           (long long)cfg_.nr * nt, ln.s),
+                    "adjoint sources");
+          ln.a = adjoint_args(shot);
+        }
+        ln.a.rec_increments = (fwd && two) ? 1 : 0;
+        ln.a.rec_off = adj_off_;
+        ln.a.rec_entries = adj_entries_;
+        for (int p = 0; p < (inplace ? 1 : 2); ++p)
+          for (int f = 0; f < 5; ++f)
+            hip_check(launch_fill(ln.st[p][f], npad_, 0.0, ln.s), "zero state");
+        ln.k = 0;
+        ln.parity = 0;
+        ln.started = true;
+      }
+      SlotFn slot = [this, shot](int it, double **dst) {
+        return stored_slot(shot, it, dst);
+      };
+      const int k1 = std::min(ln.k + chunk, nt);
+      ln.parity = enqueue_range(ln.a, ln.st, ln.parity, fwd, ln.k, k1, true, slot, nullptr,
+                                false, ln.s);
+      ln.k = k1;
+      if (ln.k == nt) {
+        if (ln.a.rec_increments)
+          hip_check(launch_receiver_scan(ln.a.rtf_x, ln.a.rtf_z, cfg_.nr, nt, ln.s),
+                    "receiver scan");
+        if (fwd) {
+          hip_check(hipEventRecord(lane_events_[shot], ln.s), "hipEventRecord");
+          ++forwards_done;
+        }
+        ln.started = false;
+        ++ln.j;
+      }
+    }
+  }
+  // join the lanes
+  hip_check(hipEventRecord(lane_events_[ns + 1], lane_stream_), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(stream_, lane_events_[ns + 1], 0), "hipStreamWaitEvent");
+  if (timed) hip_check(hipEventRecord(events_[1], stream_), "hipEventRecord");
+  if (adj) map_kernels_to_velocity();
+  calculate_l2_misfit();  // synchronises stream_
+  if (timed) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, events_[0], events_[1]), "hipEventElapsedTime");
+    timed_ms_ += ms;
+    timed_ += launches_ - before;
   }
 }
 

@@ -1,0 +1,98 @@
+"""run_model (src/fdModel.cpp:1291-1308) runs forward loops on a second HIP
+stream next to the adjoint loops of earlier shots.  Every output must be
+bitwise that of the sequential calls, for every step-kernel family, with
+and without the adjoint, and the fallbacks (checkpointing, graph mode,
+per-launch timing) must agree too."""
+import numpy as np
+import pytest
+
+from conftest import small_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _config(n_shots=4):
+    c = small_config(**{"domain.nt": 200})
+    c["sources"].update(n_sources=n_shots, n_shots=n_shots,
+                        moment_angles=[90, 45, 180, 10][:n_shots],
+                        ix_sources=[20, 51, 70, 33][:n_shots],
+                        iz_sources=[10, 7, 12, 30][:n_shots],
+                        which_source_to_fire_in_which_shot=[[k] for k in range(n_shots)])
+    return c
+
+
+def _fields(nx, nz, seed):
+    rng = np.random.default_rng(seed)
+    return (2000 + 200 * rng.random((nx, nz)), 800 + 100 * rng.random((nx, nz)),
+            1500 + 100 * rng.random((nx, nz)))
+
+
+def _prepared(cfg, variant):
+    import psvwave_amd
+    m = psvwave_amd.fdModel(cfg)
+    m.set_step_variant(variant)
+    m.set_parameter_fields(*_fields(m.nx, m.nz, 1))
+    for s in range(m.n_shots):
+        m.forward_simulate(s, store_fields=False)
+    m.set_observed_data(*m.get_synthetic_data())
+    m.set_parameter_fields(*_fields(m.nx, m.nz, 2))
+    return m
+
+
+def _sequential(m, adjoint):
+    for s in range(m.n_shots):
+        m.forward_simulate(s, store_fields=True)
+    m.calculate_l2_misfit()
+    if adjoint:
+        m.calculate_l2_adjoint_sources()
+        m.reset_kernels()
+        for s in range(m.n_shots):
+            m.adjoint_simulate(s)
+        m.map_kernels_to_velocity()
+
+
+def _outputs(m, adjoint):
+    out = [m.misfit, *m.get_synthetic_data(), *m.get_snapshots()]
+    if adjoint:
+        out += [*m.get_kernels(), m.get_gradient_vector()]
+    return out
+
+
+@pytest.mark.parametrize("variant", [9, 0, 2, 6, 14])
+@pytest.mark.parametrize("adjoint", [True, False])
+def test_two_lane_run_model_is_bitwise_sequential(variant, adjoint):
+    cfg = _config()
+    a, b = _prepared(cfg, variant), _prepared(cfg, variant)
+    a.run_model(False, adjoint)
+    _sequential(b, adjoint)
+    for k, (x, y) in enumerate(zip(_outputs(a, adjoint), _outputs(b, adjoint))):
+        assert np.array_equal(x, y), (variant, adjoint, k)
+    # a second call reuses the lane's buffers and events
+    a.run_model(False, adjoint)
+    for k, (x, y) in enumerate(zip(_outputs(a, adjoint), _outputs(b, adjoint))):
+        assert np.array_equal(x, y), (variant, adjoint, "again", k)
+
+
+def test_run_model_timing_and_fallbacks():
+    cfg = _config(3)
+    ref = _prepared(cfg, 9)
+    _sequential(ref, True)
+    want = _outputs(ref, True)
+    # loop timing brackets both lanes and counts every step of every shot
+    m = _prepared(cfg, 9)
+    m.set_timing(-1)
+    m.reset_step_timing()
+    m.run_model(False, True)
+    ms, timed, launches = m.step_timing()
+    assert timed == 2 * 3 * cfg["domain"]["nt"] and ms > 0
+    for x, y in zip(_outputs(m, True), want):
+        assert np.array_equal(x, y)
+    # per-launch timing, graph mode and checkpointing take the sequential path
+    for setup in (lambda mm: mm.set_timing(7), lambda mm: mm.set_graph_mode(True),
+                  lambda mm: mm.set_checkpointing(4)):
+        mm = _prepared(cfg, 9)
+        setup(mm)
+        mm.run_model(False, True)
+        got = [mm.misfit, *mm.get_synthetic_data(), *mm.get_kernels()]
+        assert all(np.array_equal(x, y) for x, y in
+                   zip(got, [want[0], want[1], want[2], *want[-4:-1]]))
