@@ -153,6 +153,14 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m psvwave_amd.build`"
             " (hipcc, gfx950). There is no non-HIP implementation.")
+    # One HIP runtime per process: torch ships its own copy of the HIP and
+    # HSA runtimes.  Loaded first, it is the one the library binds to (same
+    # soname); loaded after the library's /opt/rocm copy had initialised the
+    # GPU, torch would find no GPU.  So torch, when installed, comes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     for name, args in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -21,9 +21,9 @@ class _Misfit(torch.autograd.Function):
     def forward(ctx, vp, vs, rho, model):
         model.set_parameter_fields(*(t.detach().to("cpu", torch.float64).numpy()
                                      for t in (vp, vs, rho)))
-        for s in range(model.n_shots):
-            model.forward_simulate(s, store_fields=True)
-        model.calculate_l2_misfit()
+        # forward of every shot with snapshots, then the misfit
+        # (src/fdModel.cpp:1291-1297); two HIP streams, bitwise the per-shot calls
+        model.run_model(False, False)
         ctx.model = model
         return torch.tensor(model.misfit, dtype=torch.float64, device=vp.device)
 

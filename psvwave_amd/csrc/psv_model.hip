@@ -909,8 +909,8 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
   if (cfg_.nt % cfg_.snapshot_interval != 0)
     throw ValueError("Snapshot interval and size of accumulator don't match!"
                      " (nt must be a multiple of snapshot_interval)");
-  if (!verbose && !graph_mode_ && ckpt_every_ == 1 && timing_stride_ <= 0 &&
-      cfg_.n_shots > 1) {
+  if (!verbose && !graph_mode_ && (ckpt_every_ == 1 || !simulate_adjoint) &&
+      timing_stride_ <= 0 && cfg_.n_shots > 1) {
     run_model_two_lanes(simulate_adjoint);
     return;
   }
@@ -935,6 +935,8 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
 // kernel, input and accumulation order is that of the sequential calls, so
 // every output is bitwise the same.  The host enqueues the two lanes in
 // interleaved chunks of steps so that neither stream's queue runs dry.
+// (With checkpointing only the forward-only driver uses two lanes: the
+// checkpointed adjoint recomputes segments on the model stream.)
 void Model::run_model_two_lanes(bool adj) {
   const int ns = cfg_.n_shots, nt = cfg_.nt;
   ensure_snapshots();

@@ -86,9 +86,15 @@ def _simulate(model, fields, store):
         model.forward_simulate(s, store_fields=store)
 
 
-def gradient(model):
+def gradient(model, two_lanes=True):
     """Velocity kernels at the model's current parameters against its
-    observed data (the misfit is left in model.misfit)."""
+    observed data (the misfit is left in model.misfit).  two_lanes: through
+    run_model (src/fdModel.cpp:1291-1308), which overlaps the forward of the
+    next shot with the adjoint of the current one on two HIP streams; the
+    per-shot calls otherwise (bitwise the same result)."""
+    if two_lanes:
+        model.run_model(False, True)
+        return model.get_kernels()
     for s in range(model.n_shots):
         model.forward_simulate(s, store_fields=True)
     model.calculate_l2_misfit()
@@ -101,8 +107,10 @@ def gradient(model):
 
 
 def misfit_at(model, fields):
-    _simulate(model, fields, store=False)
-    model.calculate_l2_misfit()
+    """Misfit at `fields`: forward of every shot (two HIP streams, through
+    run_model without the adjoint) and the L2 misfit."""
+    model.set_parameter_fields(*fields)
+    model.run_model(False, False)
     return model.misfit
 
 
@@ -134,7 +142,7 @@ def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None):
                 _simulate(m, true, store=False)
                 m.set_observed_data(*m.get_synthetic_data())
                 m.set_parameter_fields(*bg)
-                g = gradient(m)
+                g = gradient(m, two_lanes=False)  # eight models already share the GPU
                 out_g[k] = np.stack(g).astype(np.float32)
                 out_p[k] = np.stack([(t - b) / b for t, b in zip(true, bg)]).astype(np.float32)
         except Exception as e:  # surfaced in the caller

@@ -96,3 +96,27 @@ def test_run_model_timing_and_fallbacks():
         got = [mm.misfit, *mm.get_synthetic_data(), *mm.get_kernels()]
         assert all(np.array_equal(x, y) for x, y in
                    zip(got, [want[0], want[1], want[2], *want[-4:-1]]))
+
+
+def test_forward_lanes_with_checkpointing():
+    """The forward-only driver keeps two lanes under checkpointing; the
+    checkpointed adjoint afterwards must give the sequential kernels."""
+    cfg = _config()
+    a, b = _prepared(cfg, 9), _prepared(cfg, 9)
+    for m in (a, b):
+        m.set_checkpointing(4)
+    a.run_model(False, False)
+    for s in range(b.n_shots):
+        b.forward_simulate(s, store_fields=True)
+    b.calculate_l2_misfit()
+    assert a.misfit == b.misfit
+    for x, y in zip(a.get_synthetic_data(), b.get_synthetic_data()):
+        assert np.array_equal(x, y)
+    for m in (a, b):
+        m.calculate_l2_adjoint_sources()
+        m.reset_kernels()
+        for s in range(m.n_shots):
+            m.adjoint_simulate(s)
+        m.map_kernels_to_velocity()
+    for x, y in zip(a.get_kernels(), b.get_kernels()):
+        assert np.array_equal(x, y)

@@ -2,7 +2,6 @@
 the reference's quirks, validation errors, setup arrays, and the C-ABI
 symbol table.
 """
-import ctypes
 import os
 import re
 
@@ -25,10 +24,12 @@ def test_library_exports_every_declared_symbol():
     names = set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(psv_\w+)\s*\(", header,
                            re.M))
     assert len(names) > 40
-    lib = ctypes.CDLL(os.path.join(ROOT, "psvwave_amd", "libpsvwave_amd.so"))
+    from psvwave_amd import _lib
+    lib = _lib.load()  # the package's loader (torch's HIP runtime first)
+    assert os.path.samefile(_lib.LIB_PATH, os.path.join(ROOT, "psvwave_amd",
+                                                        "libpsvwave_amd.so"))
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    from psvwave_amd import _lib
     assert names <= set(_lib._SIGNATURES), names - set(_lib._SIGNATURES)
 
 
