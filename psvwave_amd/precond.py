@@ -10,8 +10,12 @@ the surface.  Models are a smooth Gaussian-random-field background plus a
   computed by several device models at once (one host thread and one HIP
   stream each; the C-ABI releases the GIL).
 * ``UNet``: a 4-level U-Net (32 base channels) mapping the normalised
-  velocity kernels to a normalised model update; trained in bfloat16
-  autocast with fp32 master weights.
+  velocity kernels to a relative model update.  ``train_hip`` /
+  ``HipUNetTrainer`` train it and ``HipUNet`` runs it on this library's
+  bfloat16 MFMA kernels (psv_conv.hip: convolutions, input and weight
+  gradients, pooling) with fp32 accumulation and fp32 master weights;
+  ``train`` / ``learned_direction`` do the same through torch autocast, as
+  a reference point.
 * ``fwi``: steepest-descent FWI with a backtracking line search on the
   plain-forward misfit; with a preconditioner, its update direction is
   tried first and the plain gradient is the fallback when the line search
@@ -114,10 +118,14 @@ def misfit_at(model, fields):
     return model.misfit
 
 
-def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None):
+def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None, along_path=0.0):
     """n (gradient, perturbation) pairs as float32 arrays of shape
     (n, 3, nx, nz): the velocity kernels at the background model against
-    data of the true model, and (true - background) / background."""
+    data of the true model, and (true - background) / background.  With
+    along_path = f > 0, a fraction f of the pairs is taken instead at a
+    model part of the way to the true one, m = bg + t (true - bg) with
+    t ~ U(0.2, 0.8): the kernels there and the remaining (true - m) / m --
+    what the network sees in later FWI iterations."""
     from .model import fdModel
     cfg = cfg or c5_config()
     rngs = [np.random.default_rng([seed, k]) for k in range(n)]
@@ -139,6 +147,9 @@ def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None):
                 if k >= n:
                     return
                 bg, true = c5_models(rngs[k])
+                if rngs[k].random() < along_path:
+                    t = rngs[k].uniform(0.2, 0.8)
+                    bg = tuple(b + t * (tr - b) for b, tr in zip(bg, true))
                 _simulate(m, true, store=False)
                 m.set_observed_data(*m.get_synthetic_data())
                 m.set_parameter_fields(*bg)
