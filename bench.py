@@ -102,6 +102,9 @@ def _args():
     p.add_argument("--c5-pairs", type=int, default=2048)
     p.add_argument("--c5-epochs", type=int, default=20)
     p.add_argument("--c5-iterations", type=int, default=30)
+    p.add_argument("--c5-along-path", type=float, default=0.5,
+                   help="fraction of the training pairs taken part of the way "
+                        "from the background to the true model")
     p.add_argument("--c4-every", type=int, default=8)
     p.add_argument("--mc-models", type=int, default=512)
     return p.parse_args()
@@ -324,7 +327,8 @@ def learned_fwi(a):
     import psvwave_amd
     from psvwave_amd import precond
     t0 = time.perf_counter()
-    grads, perts = precond.gradient_pairs(a.c5_pairs, seed=7, workers=8, log=_progress)
+    grads, perts = precond.gradient_pairs(a.c5_pairs, seed=7, workers=8, log=_progress,
+                                          along_path=a.c5_along_path)
     t_pairs = time.perf_counter() - t0
     _progress(f"{a.c5_pairs} training pairs in {t_pairs:.1f} s")
     torch.manual_seed(0)
@@ -357,6 +361,7 @@ def learned_fwi(a):
         "data": "synthetic (seeded GRF background + perturbation)",
         "config": {"workload": "C5 learned preconditioner", "grid": "316x188", "nt": 2500,
                    "shots": 6, "receivers": 48, "training_pairs": a.c5_pairs,
+                   "pairs_along_path": a.c5_along_path,
                    "epochs": a.c5_epochs, "fwi_iterations": a.c5_iterations},
         "pairs_seconds": round(t_pairs, 1), "train_seconds": round(t_train, 1),
         "train_loss": [round(x, 5) for x in hist],

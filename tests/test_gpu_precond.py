@@ -21,6 +21,13 @@ def test_precond_pipeline_small():
     assert np.all(p[:, :, mask == 0] == 0)
     # the gradient vanishes outside the kernel region, as the reference's
     assert np.all(g[:, :, mask == 0] == 0)
+    # pairs part of the way to the true model: same models, smaller remaining
+    # perturbations, different gradients
+    g2, p2 = precond.gradient_pairs(2, seed=1, workers=2, cfg=cfg, along_path=1.0)
+    for k in range(2):
+        assert np.abs(p2[k]).max() < np.abs(p[k]).max()
+        assert np.all(np.sign(p2[k]) == np.sign(p[k]))
+        assert not np.array_equal(g2[k], g[k])
     torch.manual_seed(0)
     net = precond.make_unet(base=8)
     hist = precond.train_hip(net, g, p, epochs=3, batch=3)  # the library's kernels
