@@ -71,9 +71,9 @@ def _args():
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
                         "shot, misfit, adjoint sources, adjoint_simulate per "
-                        "shot) instead of run_model, which overlaps the "
-                        "forward of shot s + 1 with the adjoint of shot s "
-                        "on two HIP streams (bitwise the same results)")
+                        "shot) instead of run_model, which runs forward and "
+                        "adjoint loops of different shots side by side on two "
+                        "HIP streams (bitwise the same results)")
     p.add_argument("--graph", action="store_true",
                    help="replay each shot's time loop as a captured HIP graph")
     p.add_argument("--timing-stride", type=int, default=-1,
@@ -590,9 +590,10 @@ def main():
             "hip_graph": a.graph,
             "driver": ("call by call, one stream" if a.sequential or a.graph or
                        a.timing_stride > 0 else
-                       "run_model: forward of shot s+1 on a second HIP stream "
-                       "beside the adjoint of shot s (bitwise the sequential "
-                       "results)"),
+                       "run_model on two HIP streams: forwards of shots 1.. "
+                       "beside the adjoints of shots 0.., the last adjoint's "
+                       "kernel terms deferred and replayed in order (bitwise "
+                       "the sequential results)"),
         },
         "roofline": {
             "bound": "hbm",

@@ -290,6 +290,11 @@ hipError_t launch_adjoint_sources(const double *sx, const double *sz,
                                   const double *ox, const double *oz,
                                   double *ax, double *az, long long n,
                                   hipStream_t s);
+// Adds deferred correlation terms [slot][rho, lam, mu][nx * nz] to the
+// kernels over the kernel region, slots from last to first.
+hipError_t launch_replay_correlation(double *rho_k, double *lam_k, double *mu_k,
+                                     const double *d, int nslots, long long ngrid, int nx,
+                                     int kx0, int kx1, int kz0, int kz1, hipStream_t s);
 hipError_t launch_map_kernels(const double *vp, const double *vs,
                               const double *rho, const double *lam_k,
                               const double *mu_k, const double *rho_l,

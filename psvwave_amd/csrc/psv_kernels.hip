@@ -2120,6 +2120,42 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
   return hipGetLastError();
 }
 
+// Deferred kernel correlation: an adjoint whose kernel pointers pointed at
+// a zeroed buffer per snapshot slot ([slot][rho, lam, mu][nx * nz]) left there
+// exactly the terms it would have applied (0 - e = -e for the density kernel,
+// 0 + e = e for the others).  Adding them in the adjoint's own order -- slots
+// from last to first -- repeats its roundings: K - e == K + (-e).  Only the
+// kernel region [kx0, kx1) x [kz0, kz1) is ever correlated.
+__global__ void replay_correlation_kernel(double *rho_k, double *lam_k, double *mu_k,
+                                          const double *__restrict__ d, int nslots,
+                                          long long ngrid, int nx, int kx0, int kx1,
+                                          int kz0, int kz1) {
+  const long long w = kx1 - kx0;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w * (kz1 - kz0)) return;
+  const long long ci = (long long)(kz0 + i / w) * nx + kx0 + i % w;
+  double r = rho_k[ci], l = lam_k[ci], m = mu_k[ci];
+  for (int s = nslots - 1; s >= 0; --s) {
+    const double *ds = d + (long long)s * 3 * ngrid + ci;
+    r = r + ds[0];
+    l = l + ds[ngrid];
+    m = m + ds[2 * ngrid];
+  }
+  rho_k[ci] = r;
+  lam_k[ci] = l;
+  mu_k[ci] = m;
+}
+
+hipError_t launch_replay_correlation(double *rho_k, double *lam_k, double *mu_k,
+                                     const double *d, int nslots, long long ngrid, int nx,
+                                     int kx0, int kx1, int kz0, int kz1, hipStream_t s) {
+  const long long n = (long long)(kx1 - kx0) * (kz1 - kz0);
+  if (n <= 0 || nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(replay_correlation_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     rho_k, lam_k, mu_k, d, nslots, ngrid, nx, kx0, kx1, kz0, kz1);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   long long nb = (n + 255) / 256;

@@ -157,7 +157,7 @@ class Model {
   int enqueue_range(StepArgs &a, double *(*st)[5], int parity, bool forward,
                     int k0, int k1, bool snaps, const SlotFn &slot,
                     std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
-                    bool dump, hipStream_t s = nullptr);
+                    bool dump, hipStream_t s = nullptr, double *defer = nullptr);
   StepArgs forward_args(int i_shot) const;
   StepArgs adjoint_args(int i_shot) const;
   // run_model with two lanes: forward time loops on lane_stream_ (state in
@@ -167,6 +167,9 @@ class Model {
   hipStream_t lane_stream_ = nullptr;
   double *lane_state_[2][5] = {};
   std::vector<hipEvent_t> lane_events_;
+  double *defer_ = nullptr;  // deferred correlation terms [slot][3][ngrid]
+  long long defer_n_ = 0;
+  bool ensure_defer_buffer();
   void enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
                     std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
                     bool dump);

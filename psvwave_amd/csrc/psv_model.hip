@@ -482,7 +482,7 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
                          bool forward, int k0, int k1, bool snaps,
                          const SlotFn &slot,
                          std::vector<std::pair<hipEvent_t, hipEvent_t>> *pairs,
-                         bool dump, hipStream_t s) {
+                         bool dump, hipStream_t s, double *defer) {
   hipStream_t ls = s ? s : stream_;
   const bool inplace = step_is_inplace(variant_);
   const bool two = step_is_two_step(variant_) && !dump;
@@ -517,6 +517,14 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
     a.do_snap = snaps && (it % si == 0) && slot(it, a.snap);
     a.it1 = it1;
     a.do_snap1 = pair && snaps && (it1 % si == 0) && slot(it1, a.snap1);
+    if (defer && a.do_snap && !forward) {
+      // deferred correlation (one-step kernels only): this snapshot slot's
+      // terms go to their own zeroed slice instead of the kernels
+      const long long off = (long long)(it / si) * 3 * ngrid_;
+      a.rho_k = defer + off;
+      a.lam_k = defer + off + ngrid_;
+      a.mu_k = defer + off + 2 * ngrid_;
+    }
     const int covered = pair ? 2 : 1;
     bool timed = pairs && timing_stride_ > 0 && (launches_ / covered) % timing_stride_ == 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -924,16 +932,37 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
   }
 }
 
+// Buffer for the deferred correlation of one adjoint: 3 arrays per snapshot
+// slot.  Allocated once when the device has room for it (and a margin).
+bool Model::ensure_defer_buffer() {
+  const long long n = (long long)d_.snapshots * 3 * ngrid_;
+  if (defer_ && defer_n_ == n) return true;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  const double need = (double)n * sizeof(double) + (double)(2ull << 30);
+  if ((double)free_b < need) return false;
+  defer_ = dalloc((size_t)n);
+  defer_n_ = n;
+  return true;
+}
+
 // run_model on two HIP streams.  A single time loop leaves the GPU partly
-// idle at the end of every launch (the last blocks of a launch run while
-// the next launch, which depends on them, waits); two independent loops on
-// two streams fill each other's gaps.  Lane 1 (lane_stream_, state in
-// lane_state_) runs the forward loops of all shots; lane 0 (stream_, state_)
-// runs the adjoint of shot s after the forward of shot s, in shot order, so
-// the kernel accumulation happens in the same order as one shot at a time
-// (without the adjoint, the two lanes forward alternate shots).  Every
-// kernel, input and accumulation order is that of the sequential calls, so
-// every output is bitwise the same.  The host enqueues the two lanes in
+// idle between launches (the completion-to-dispatch latency of dependent
+// launches and the last blocks of each launch); two independent loops on
+// two streams fill each other's gaps.  Lane X (stream_, state_) and lane Y
+// (lane_stream_, lane_state_) run, for one-step kernels:
+//   X: forward 0, adjoint 0, 1, ..., n-2, replay of adjoint n-1
+//   Y: forward 1, 2, ..., n-1, adjoint n-1 (deferred)
+// so every time loop has a partner.  The adjoints of shots 0 .. n-2
+// accumulate the kernels in shot order; the last one runs beside them with
+// its kernel pointers aimed at a zeroed buffer per snapshot slot, which
+// therefore receives exactly the terms it would have applied, and the replay
+// adds them to the kernels in the adjoint's own order after adjoint n-2
+// (K - e == K + (0 - e) in IEEE arithmetic).  With two-step kernels (two
+// correlation steps may share a launch) or without room for the buffer:
+//   X: adjoint 0, ..., n-1 (each after its forward);  Y: forward 0, ..., n-1.
+// Without the adjoint the lanes forward alternate shots.  Every output is
+// bitwise that of the sequential calls.  The host enqueues the lanes in
 // interleaved chunks of steps so that neither stream's queue runs dry.
 // (With checkpointing only the forward-only driver uses two lanes: the
 // checkpointed adjoint recomputes segments on the model stream.)
@@ -946,7 +975,7 @@ void Model::run_model_two_lanes(bool adj) {
   if (!lane_state_[0][0])
     for (int p = 0; p < 2; ++p)
       for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
-  while ((int)lane_events_.size() < ns + 2) {
+  while ((int)lane_events_.size() < ns + 3) {
     hipEvent_t e;
     hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     lane_events_.push_back(e);
@@ -959,6 +988,7 @@ void Model::run_model_two_lanes(bool adj) {
   const bool timed = timing_stride_ < 0;
   const bool inplace = step_is_inplace(variant_);
   const bool two = step_is_two_step(variant_);
+  const bool balanced = adj && !two && ensure_defer_buffer();
   const int64_t before = launches_;
   if (timed) hip_check(hipEventRecord(events_[0], stream_), "hipEventRecord");
   if (adj) {
@@ -966,53 +996,89 @@ void Model::run_model_two_lanes(bool adj) {
     hip_check(launch_fill(mu_k_, ngrid_, 0.0, stream_), "reset");
     hip_check(launch_fill(rho_l_, ngrid_, 0.0, stream_), "reset");
   }
-  // the second lane starts after everything enqueued so far on stream_
+  // lane Y starts after everything enqueued so far on stream_
   hip_check(hipEventRecord(lane_events_[ns], stream_), "hipEventRecord");
   hip_check(hipStreamWaitEvent(lane_stream_, lane_events_[ns], 0), "hipStreamWaitEvent");
 
+  enum Kind { kFwd, kAdj, kAdjDeferred, kReplay };
+  struct Job {
+    int shot;
+    Kind kind;
+  };
   struct Lane {
     hipStream_t s;
     double *(*st)[5];
-    std::vector<std::pair<int, bool>> jobs;  // (shot, forward)
+    std::vector<Job> jobs;
     size_t j = 0;
     int k = 0, parity = 0;
     bool started = false;
     StepArgs a;
   };
-  Lane lanes[2];
+  Lane lanes[2];  // 0: X, 1: Y
   lanes[0].s = stream_;
   lanes[0].st = state_;
   lanes[1].s = lane_stream_;
   lanes[1].st = lane_state_;
-  for (int sh = 0; sh < ns; ++sh) {
-    if (adj) {
-      lanes[1].jobs.emplace_back(sh, true);
-      lanes[0].jobs.emplace_back(sh, false);
-    } else {
-      lanes[1 - (sh & 1)].jobs.emplace_back(sh, true);
+  std::vector<int> fwd_lane(ns, -1);  // lane of each shot's forward
+  if (balanced) {
+    lanes[0].jobs.push_back({0, kFwd});
+    for (int sh = 1; sh < ns; ++sh) lanes[1].jobs.push_back({sh, kFwd});
+    for (int sh = 0; sh + 1 < ns; ++sh) lanes[0].jobs.push_back({sh, kAdj});
+    lanes[1].jobs.push_back({ns - 1, kAdjDeferred});
+    lanes[0].jobs.push_back({ns - 1, kReplay});
+  } else {
+    for (int sh = 0; sh < ns; ++sh) {
+      if (adj) {
+        lanes[1].jobs.push_back({sh, kFwd});
+        lanes[0].jobs.push_back({sh, kAdj});
+      } else {
+        lanes[1 - (sh & 1)].jobs.push_back({sh, kFwd});
+      }
     }
   }
-  int forwards_done = 0;  // forward loops fully enqueued (lane 1 with the adjoint)
+  for (int l = 0; l < 2; ++l)
+    for (const Job &jb : lanes[l].jobs)
+      if (jb.kind == kFwd) fwd_lane[jb.shot] = l;
+  std::vector<char> fwd_enqueued(ns, 0);
+  bool deferred_enqueued = false;
   const int chunk = 64;
   while (lanes[0].j < lanes[0].jobs.size() || lanes[1].j < lanes[1].jobs.size()) {
     for (int l = 1; l >= 0; --l) {
       Lane &ln = lanes[l];
       if (ln.j >= ln.jobs.size()) continue;
-      const int shot = ln.jobs[ln.j].first;
-      const bool fwd = ln.jobs[ln.j].second;
-      if (!fwd && shot >= forwards_done) continue;  // its forward is not enqueued yet
+      const Job jb = ln.jobs[ln.j];
+      const bool fwd = jb.kind == kFwd;
+      if ((jb.kind == kAdj || jb.kind == kAdjDeferred) && !fwd_enqueued[jb.shot]) continue;
+      if (jb.kind == kReplay && !deferred_enqueued) continue;
+      if (jb.kind == kReplay) {
+        // after adjoint n-2 (this lane) and the deferred adjoint (lane Y)
+        hip_check(hipStreamWaitEvent(ln.s, lane_events_[ns + 2], 0), "hipStreamWaitEvent");
+        hip_check(launch_replay_correlation(
+                      rho_l_, lam_k_, mu_k_, defer_, d_.snapshots, ngrid_, d_.nx,
+                      cfg_.np_boundary + cfg_.nx_inner_boundary,
+                      cfg_.np_boundary + cfg_.nx_inner - cfg_.nx_inner_boundary,
+                      cfg_.np_boundary + cfg_.nz_inner_boundary,
+                      cfg_.np_boundary + cfg_.nz_inner - cfg_.nz_inner_boundary, ln.s),
+                  "replay correlation");
+        ++ln.j;
+        continue;
+      }
+      const int shot = jb.shot;
       if (!ln.started) {
         if (fwd) {
           ln.a = forward_args(shot);
         } else {
           // the adjoint source of this shot once its forward has run
-          hip_check(hipStreamWaitEvent(ln.s, lane_events_[shot], 0), "hipStreamWaitEvent");
+          if (fwd_lane[shot] != l)
+            hip_check(hipStreamWaitEvent(ln.s, lane_events_[shot], 0), "hipStreamWaitEvent");
           const size_t off = (size_t)shot * cfg_.nr * nt;
           hip_check(launch_adjoint_sources(syn_[0] + off, syn_[1] + off, obs_[0] + off,
                                            obs_[1] + off, adj_[0] + off, adj_[1] + off,
                                            (long long)cfg_.nr * nt, ln.s),
                     "adjoint sources");
           ln.a = adjoint_args(shot);
+          if (jb.kind == kAdjDeferred)
+            hip_check(launch_fill(defer_, defer_n_, 0.0, ln.s), "zero deferred terms");
         }
         ln.a.rec_increments = (fwd && two) ? 1 : 0;
         ln.a.rec_off = adj_off_;
@@ -1029,7 +1095,7 @@ void Model::run_model_two_lanes(bool adj) {
       };
       const int k1 = std::min(ln.k + chunk, nt);
       ln.parity = enqueue_range(ln.a, ln.st, ln.parity, fwd, ln.k, k1, true, slot, nullptr,
-                                false, ln.s);
+                                false, ln.s, jb.kind == kAdjDeferred ? defer_ : nullptr);
       ln.k = k1;
       if (ln.k == nt) {
         if (ln.a.rec_increments)
@@ -1037,7 +1103,11 @@ void Model::run_model_two_lanes(bool adj) {
                     "receiver scan");
         if (fwd) {
           hip_check(hipEventRecord(lane_events_[shot], ln.s), "hipEventRecord");
-          ++forwards_done;
+          fwd_enqueued[shot] = 1;
+        }
+        if (jb.kind == kAdjDeferred) {
+          hip_check(hipEventRecord(lane_events_[ns + 2], ln.s), "hipEventRecord");
+          deferred_enqueued = true;
         }
         ln.started = false;
         ++ln.j;
