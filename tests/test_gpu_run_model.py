@@ -1,8 +1,9 @@
-"""run_model (src/fdModel.cpp:1291-1308) runs forward loops on a second HIP
-stream next to the adjoint loops of earlier shots.  Every output must be
-bitwise that of the sequential calls, for every step-kernel family, with
-and without the adjoint, and the fallbacks (checkpointing, graph mode,
-per-launch timing) must agree too."""
+"""run_model (src/fdModel.cpp:1291-1308) runs forward and adjoint loops of
+different shots on two HIP streams; with one-step kernels the last adjoint
+defers its kernel terms to a buffer that is replayed in order.  Every output
+must be bitwise that of the sequential calls, for every step-kernel family,
+with and without the adjoint, for two and four shots, and the fallbacks
+(checkpointing, graph mode, per-launch timing) must agree too."""
 import numpy as np
 import pytest
 
@@ -60,8 +61,9 @@ def _outputs(m, adjoint):
 
 @pytest.mark.parametrize("variant", [9, 0, 2, 6, 14])
 @pytest.mark.parametrize("adjoint", [True, False])
-def test_two_lane_run_model_is_bitwise_sequential(variant, adjoint):
-    cfg = _config()
+@pytest.mark.parametrize("n_shots", [4, 2])
+def test_two_lane_run_model_is_bitwise_sequential(variant, adjoint, n_shots):
+    cfg = _config(n_shots)
     a, b = _prepared(cfg, variant), _prepared(cfg, variant)
     a.run_model(False, adjoint)
     _sequential(b, adjoint)
