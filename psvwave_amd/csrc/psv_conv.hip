@@ -67,6 +67,24 @@ __device__ inline long long conv_src(const ConvGemm &a, int tap, int yg, int x, 
   return ((long long)(yg + dy) * a.W + x + dx) * a.in_stride;
 }
 
+// Epilogue of one output value (bias already added): keep the rounded
+// pre-activation (training), GELU, times the GELU derivative of a stored
+// pre-activation (backward), store.
+__device__ inline void conv_store(const ConvGemm &a, long long pix, int co, float v) {
+  if (a.pre16) {
+    const uint16_t h = to_bf16(v);
+    a.pre16[pix * a.cout + co] = h;
+    v = from_bf16(h);
+  }
+  if (a.gelu) v = gelu(v);
+  if (a.dz) v *= gelu_grad(from_bf16(a.dz[pix * a.dz_stride + co]));
+  const long long o = pix * a.out_stride + a.out_off + co;
+  if (a.out32)
+    a.out32[o] = v;
+  else
+    a.out16[o] = to_bf16(v);
+}
+
 // One wavefront computes a (32 WM) x (32 WN) tile of out with
 // v_mfma_f32_32x32x16_bf16: per step of 16 along K each lane loads 16 bytes
 // per 32-row slice of A and of B, and issues WM x WN matrix instructions.
@@ -154,20 +172,78 @@ __global__ void __launch_bounds__(256) conv_mfma_kernel(ConvGemm a) {
           const int q = n / a.cout, py = mm / a.W, px = mm - py * a.W;
           pix = (long long)(2 * py + (q >> 1)) * (2 * a.W) + (2 * px + (q & 1));
         }
-        float v = acc[i][j][r] + b;
-        if (a.pre16) {  // training: keep the rounded pre-activation for the backward pass
-          const uint16_t h = to_bf16(v);
-          a.pre16[pix * a.cout + co] = h;
-          v = from_bf16(h);
-        }
-        if (a.gelu) v = gelu(v);
-        if (a.dz) v *= gelu_grad(from_bf16(a.dz[pix * a.dz_stride + co]));
-        const long long o = pix * a.out_stride + a.out_off + co;
-        if (a.out32)
-          a.out32[o] = v;
-        else
-          a.out16[o] = to_bf16(v);
+        conv_store(a, pix, co, acc[i][j][r] + b);
       }
+    }
+  }
+}
+
+// 3 x 3 convolution (mode 0, cin a multiple of 16, H a multiple of 4) with
+// the operands staged in LDS.  A block computes 4 output rows x 32 columns
+// x (32 WN) channels; each wavefront one row.  Per 16 input channels the
+// block loads the input tile with its halo (6 x 34 pixels, zero outside the
+// image) and the weights of all 9 taps for its channels into LDS; the 9
+// taps then read their A fragments from the same tile shifted by (dy, dx),
+// so every input value is fetched from L2 once per block instead of 9 times.
+constexpr int kC3Rows = 4, kC3Cols = 32, kC3TileW = kC3Cols + 2, kC3TileP = (kC3Rows + 2) * kC3TileW;
+template <int WN>
+__global__ void __launch_bounds__(256) conv3_lds_kernel(ConvGemm a) {
+  __shared__ __attribute__((aligned(16))) uint16_t s_in[kC3TileP * 16];
+  __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * 32 * WN * 16];
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, half = lane >> 5, l32 = lane & 31;
+  const int N = a.cout, NB = 32 * WN, K = 9 * a.cin;
+  const int ntn = (N + NB - 1) / NB, ncol = (a.W + kC3Cols - 1) / kC3Cols;
+  const int nt = blockIdx.x % ntn, rest = blockIdx.x / ntn;
+  const int ct = rest % ncol, band = rest / ncol;
+  const int y0 = band * kC3Rows, x0 = ct * kC3Cols, n0 = nt * NB;
+  const int yi0 = y0 % a.H;  // image row of the band's first row (bands never straddle images)
+  f32x16 acc[WN];
+#pragma unroll
+  for (int j = 0; j < WN; ++j)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[j][r] = 0.f;
+  const u32x4 zero = {0u, 0u, 0u, 0u};
+  for (int c0 = 0; c0 < a.cin; c0 += 16) {
+    __syncthreads();  // the previous chunk's fragments have been read
+    for (int item = t; item < kC3TileP * 2; item += 256) {
+      const int p = item >> 1, h = item & 1, r = p / kC3TileW, c = p - r * kC3TileW;
+      const int yl = yi0 - 1 + r, x = x0 - 1 + c;
+      u32x4 v = zero;
+      if (yl >= 0 && yl < a.H && x >= 0 && x < a.W)
+        v = *(const u32x4 *)(a.in + ((long long)(y0 - 1 + r) * a.W + x) * a.in_stride + c0 +
+                             8 * h);
+      *(u32x4 *)(s_in + p * 16 + 8 * h) = v;
+    }
+    for (int item = t; item < 9 * NB * 2; item += 256) {
+      const int h = item & 1, tn = item >> 1, tap = tn / NB, n = tn - tap * NB;
+      u32x4 v = zero;
+      if (n0 + n < N)
+        v = *(const u32x4 *)(a.w + (long long)(n0 + n) * K + tap * a.cin + c0 + 8 * h);
+      *(u32x4 *)(s_w + tn * 16 + 8 * h) = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;
+      const bf16x8 av =
+          *(const bf16x8 *)(s_in + ((wave + dy) * kC3TileW + l32 + dx) * 16 + 8 * half);
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        const bf16x8 bv = *(const bf16x8 *)(s_w + (tap * NB + 32 * j + l32) * 16 + 8 * half);
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[j], 0, 0, 0);
+      }
+    }
+  }
+  const int y = y0 + wave;
+#pragma unroll
+  for (int j = 0; j < WN; ++j) {
+    const int n = n0 + 32 * j + l32;
+    if (n >= N) continue;
+    const float b = a.bias ? a.bias[n] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int x = x0 + acc_row(r, half);
+      if (x < a.W) conv_store(a, (long long)y * a.W + x, n, acc[j][r] + b);
     }
   }
 }
@@ -421,6 +497,17 @@ hipError_t launch_conv_gemm(const ConvGemm &a, hipStream_t s) {
       a.mode < kConv || a.mode > kGather2x2 || (a.pre16 && a.mode != kConv))
     return hipErrorInvalidValue;
   if (((uintptr_t)a.in | (uintptr_t)a.w) & 15) return hipErrorInvalidValue;  // 16-byte loads
+  if (a.mode == kConv && a.ks == 3 && a.cin % 16 == 0 && a.H % kC3Rows == 0) {
+    const long long blocks = (long long)(a.rows / kC3Rows) * ((a.W + kC3Cols - 1) / kC3Cols);
+    if (a.cout > 32) {
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3_lds_kernel<2>),
+                         dim3((unsigned)(blocks * ((a.cout + 63) / 64))), dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(conv3_lds_kernel<1>),
+                         dim3((unsigned)(blocks * ((a.cout + 31) / 32))), dim3(256), 0, s, a);
+    }
+    return hipGetLastError();
+  }
   const long long M = (long long)a.rows * a.W, N = a.mode == kScatter ? 4LL * a.cout : a.cout;
   if (N > 32) {  // 64 x 64 per wavefront
     const long long tiles = ((M + 63) / 64) * ((N + 63) / 64);
