@@ -180,8 +180,9 @@ class Model {
   int stored_snapshots() const;
   int ckpt_every_ = 1;
   double *state2_[2][5] = {};  // forward recomputation buffers (padded)
-  double *seg_[5] = {};        // snapshots of one recomputed segment
-  int seg_k_ = 0;              // slots allocated in seg_
+  double *seg_[5] = {};        // snapshots of two recomputed segments
+  int seg_k_ = 0;              // slots per segment buffer in seg_
+  std::vector<hipEvent_t> ckpt_events_;
   int *scratch_off_ = nullptr; // all-zero CSR offsets
 };
 

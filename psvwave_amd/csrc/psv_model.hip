@@ -92,6 +92,7 @@ Model::~Model() {
   clear_graphs();
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : lane_events_) (void)hipEventDestroy(e);
+  for (auto e : ckpt_events_) (void)hipEventDestroy(e);
   if (lane_stream_) (void)hipStreamDestroy(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
@@ -608,8 +609,9 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
 // Adjoint with checkpointing: segments of ckpt_every_ snapshot intervals
 // are processed last to first.  For each, the forward state is restored
 // from the segment's checkpoint into a second buffer pair, the segment's
-// snapshots are recomputed into a small buffer, and the adjoint steps of
-// the segment correlate with them.  The recomputed forward runs the same
+// snapshots are recomputed into one of two small buffers (on the second
+// stream, one segment ahead of the adjoint), and the adjoint steps of the
+// segment correlate with them.  The recomputed forward runs the same
 // kernels on the same state, so the kernels equal those of full storage
 // bit for bit.
 void Model::enqueue_checkpointed_adjoint(
@@ -623,22 +625,35 @@ void Model::enqueue_checkpointed_adjoint(
     const long long max_tiles = (long long)ntx_ * ((d_.nz + 7) / 8);
     scratch_off_ = (int *)dalloc((size_t)(max_tiles + 2) / 2 + 1);
   }
-  if (seg_k_ < k) {
+  if (seg_k_ < k) {  // two segment buffers of k snapshot slots
     for (int f = 0; f < 5; ++f) {
       if (seg_[f]) {
         auto it = std::find(allocations_.begin(), allocations_.end(), (void *)seg_[f]);
         if (it != allocations_.end()) allocations_.erase(it);
         hip_check(hipFree(seg_[f]), "hipFree");
-        device_bytes_ -= (int64_t)((long long)seg_k_ * ngrid_ * sizeof(double));
+        device_bytes_ -= (int64_t)(2LL * seg_k_ * ngrid_ * sizeof(double));
       }
-      seg_[f] = dalloc((size_t)k * ngrid_);
+      seg_[f] = dalloc((size_t)2 * k * ngrid_);
     }
     seg_k_ = k;
   }
+  if (!lane_stream_)
+    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
+              "hipStreamCreate");
+  while (ckpt_events_.size() < 6) {
+    hipEvent_t e;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    ckpt_events_.push_back(e);
+  }
+  hipEvent_t ev_start = ckpt_events_[0], ev_join = ckpt_events_[1];
+  hipEvent_t ev_ready[2] = {ckpt_events_[2], ckpt_events_[3]};  // segment recomputed
+  hipEvent_t ev_done[2] = {ckpt_events_[4], ckpt_events_[5]};   // segment correlated
   const bool inplace = step_is_inplace(variant_);
   for (int p = 0; p < (inplace ? 1 : 2); ++p)
     for (int f = 0; f < 5; ++f)
       hip_check(launch_fill(state_[p][f], npad_, 0.0, stream_), "zero state");
+  hip_check(hipEventRecord(ev_start, stream_), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(lane_stream_, ev_start, 0), "hipStreamWaitEvent");
   StepArgs fa = base_args();
   fa.adjoint = 0;
   fa.dts = cfg_.dt;
@@ -649,26 +664,37 @@ void Model::enqueue_checkpointed_adjoint(
   fa.nr = 0;  // the recomputation records no receivers
   fa.rec_off = scratch_off_;  // all-zero CSR: no entries
   fa.rec_entries = adj_entries_;
+  // Segment c is recomputed on lane_stream_ into buffer c % 2 while the
+  // adjoint of segment c + 1 runs on stream_ from the other buffer; the
+  // adjoint steps stay in order on stream_, so the kernels are those of one
+  // stream bit for bit.
   const int L = k * si;
   const int nseg = (nt + L - 1) / L;
   int parity = 0;
   for (int c = nseg - 1; c >= 0; --c) {
-    const int t0 = c * L, t1 = std::min(t0 + L, nt);
-    SlotFn seg = [this, c, k, si](int it, double **dst) {
-      long long off = (long long)(it / si - c * k) * ngrid_;
+    const int t0 = c * L, t1 = std::min(t0 + L, nt), b = c & 1;
+    SlotFn seg = [this, c, k, si, b](int it, double **dst) {
+      long long off = ((long long)b * k + (it / si - c * k)) * ngrid_;
       for (int f = 0; f < 5; ++f) dst[f] = seg_[f] + off;
       return true;
     };
+    if (c + 2 < nseg)  // buffer b was read by the adjoint of segment c + 2
+      hip_check(hipStreamWaitEvent(lane_stream_, ev_done[b], 0), "hipStreamWaitEvent");
     double *ck[5];
     stored_slot(i_shot, t0, ck);
     for (int f = 0; f < 5; ++f)
       hip_check(launch_pad(ck[f], state2_[0][f], d_.nx, d_.nz, pitch_, origin_,
-                           stream_),
+                           lane_stream_),
                 "restore checkpoint");
-    enqueue_range(fa, state2_, 0, true, t0, t1, true, seg, nullptr, false);
+    enqueue_range(fa, state2_, 0, true, t0, t1, true, seg, nullptr, false, lane_stream_);
+    hip_check(hipEventRecord(ev_ready[b], lane_stream_), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(stream_, ev_ready[b], 0), "hipStreamWaitEvent");
     parity = enqueue_range(aa, state_, parity, false, nt - t1, nt - t0, true,
                            seg, pairs, false);
+    hip_check(hipEventRecord(ev_done[b], stream_), "hipEventRecord");
   }
+  hip_check(hipEventRecord(ev_join, lane_stream_), "hipEventRecord");
+  hip_check(hipStreamWaitEvent(stream_, ev_join, 0), "hipStreamWaitEvent");
 }
 
 // The time loop.  Plain mode launches step by step; graph mode replays a
