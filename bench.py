@@ -105,6 +105,10 @@ def _args():
     p.add_argument("--c5-along-path", type=float, default=0.5,
                    help="fraction of the training pairs taken part of the way "
                         "from the background to the true model")
+    p.add_argument("--c5-no-mirror", action="store_true",
+                   help="use the network's prediction alone instead of its "
+                        "mean with the mirrored prediction of the mirrored "
+                        "gradient")
     p.add_argument("--c4-every", type=int, default=8)
     p.add_argument("--mc-models", type=int, default=512)
     return p.parse_args()
@@ -348,7 +352,8 @@ def learned_fwi(a):
                              log=lambda m: _progress("plain " + m))
     target = h_plain[-1][1]
     h_learn, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
-                             direction=lambda g, f: precond.learned_direction_hip(unet, g, f),
+                             direction=lambda g, f: precond.learned_direction_hip(
+                                 unet, g, f, mirror=not a.c5_no_mirror),
                              target=target, log=lambda m: _progress("learned " + m))
     t_plain = precond.time_to_misfit(h_plain, target)
     t_learn = precond.time_to_misfit(h_learn, target)
@@ -361,7 +366,7 @@ def learned_fwi(a):
         "data": "synthetic (seeded GRF background + perturbation)",
         "config": {"workload": "C5 learned preconditioner", "grid": "316x188", "nt": 2500,
                    "shots": 6, "receivers": 48, "training_pairs": a.c5_pairs,
-                   "pairs_along_path": a.c5_along_path,
+                   "pairs_along_path": a.c5_along_path, "mirror_average": not a.c5_no_mirror,
                    "epochs": a.c5_epochs, "fwi_iterations": a.c5_iterations},
         "pairs_seconds": round(t_pairs, 1), "train_seconds": round(t_train, 1),
         "train_loss": [round(x, 5) for x in hist],

@@ -451,9 +451,15 @@ class HipUNet:
         return y[:nx, :nz].permute(2, 0, 1).contiguous()
 
 
-def learned_direction_hip(unet, g, fields):
-    """As learned_direction, with the network run by HipUNet."""
-    y = unet(normalise_gradient(np.stack(g))).cpu().numpy().astype(np.float64)
+def learned_direction_hip(unet, g, fields, mirror=False):
+    """As learned_direction, with the network run by HipUNet.  mirror: the
+    average of the prediction and the mirrored prediction for the mirrored
+    gradient (the network is trained with mirror augmentation in x)."""
+    x = normalise_gradient(np.stack(g))
+    y = unet(x).cpu().numpy().astype(np.float64)
+    if mirror:
+        ym = unet(np.ascontiguousarray(x[:, ::-1])).cpu().numpy().astype(np.float64)
+        y = 0.5 * (y + ym[:, ::-1])
     return [yy * f for yy, f in zip(y, fields)]
 
 

@@ -295,3 +295,23 @@ def test_unet_training_gradients_match_torch():
     rel, rel16 = (tot / totref) ** 0.5, (tot16 / totref) ** 0.5
     assert abs(float(loss) - float(ref_loss)) < 0.02 * float(ref_loss)
     assert rel < 3 * rel16 + 0.01, (rel, rel16)
+
+
+def test_mirror_averaged_direction():
+    """learned_direction_hip(mirror=True): mean of the prediction and the
+    mirrored prediction of the mirrored gradient, times the fields."""
+    import torch
+    from psvwave_amd import precond
+    torch.manual_seed(2)
+    hip = precond.HipUNet(precond.make_unet(base=8).cuda().eval())
+    rng = np.random.default_rng(7)
+    g = [rng.standard_normal((40, 30)) for _ in range(3)]
+    fields = [1000 + rng.random((40, 30)) for _ in range(3)]
+    x = precond.normalise_gradient(np.stack(g))
+    y = hip(x).cpu().numpy().astype(np.float64)
+    ym = hip(np.ascontiguousarray(x[:, ::-1])).cpu().numpy().astype(np.float64)[:, ::-1]
+    d = precond.learned_direction_hip(hip, g, fields, mirror=True)
+    for k in range(3):
+        assert np.array_equal(d[k], 0.5 * (y[k] + ym[k]) * fields[k])
+    d0 = precond.learned_direction_hip(hip, g, fields)
+    assert all(np.array_equal(a, b * f) for a, b, f in zip(d0, y, fields))
