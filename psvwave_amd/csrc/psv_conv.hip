@@ -29,7 +29,6 @@ namespace psv {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -521,7 +520,7 @@ hipError_t launch_conv_gemm(const ConvGemm &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-void launch_reduce_splits(const float *part, float *out, long long n, int nsplit,
+static void launch_reduce_splits(const float *part, float *out, long long n, int nsplit,
                           hipStream_t s) {
   int so = 1;
   while (so < 256 && (long long)so * 8 < nsplit) so *= 2;
@@ -529,7 +528,7 @@ void launch_reduce_splits(const float *part, float *out, long long n, int nsplit
                      part, out, n, nsplit, so);
 }
 
-long long wgrad_plan(Wgrad &a, long long workspace_floats) {
+static long long wgrad_plan(Wgrad &a, long long workspace_floats) {
   const long long P = (long long)a.rows * a.W;
   const long long tiles = (long long)((a.I + 63) / 64) * ((a.J + 63) / 64);
   long long ns = (2048 + tiles - 1) / tiles;
