@@ -18,6 +18,7 @@ from the committed rocprofv3 counter summary for the kernel.
 bounded sample of the same grid.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -247,6 +248,10 @@ def gradient_check(a):
     grads = {}
     obs = None
     for every in (1, a.c4_every):
+        # release the previous model first: its streams would share the
+        # runtime's hardware queues with this one's two lanes
+        m = f = params = None
+        gc.collect()
         m = psvwave_amd.fdModel(cfg)
         m.set_checkpointing(every)
         if obs is None:
@@ -255,16 +260,20 @@ def gradient_check(a):
                 m.forward_simulate(s, store_fields=False)
             obs = m.get_synthetic_data()
         m.set_observed_data(*obs)
-        params = [torch.tensor(f, dtype=torch.float64, requires_grad=True) for f in start]
-        torch.cuda.synchronize() if torch.cuda.is_available() else None
-        t0 = time.perf_counter()
-        f = misfit(*params, m)
-        f.backward()
-        secs = time.perf_counter() - t0
+        times = []
+        for rep in range(2):  # the first call allocates snapshot and lane buffers
+            params = [torch.tensor(f, dtype=torch.float64, requires_grad=True) for f in start]
+            torch.cuda.synchronize() if torch.cuda.is_available() else None
+            t0 = time.perf_counter()
+            f = misfit(*params, m)
+            f.backward()
+            times.append(time.perf_counter() - t0)
+        secs = times[1]
         grads[every] = [p.grad.numpy().copy() for p in params]
-        res[every] = {"seconds": round(secs, 3), "device_bytes": m.device_bytes(),
-                      "misfit": float(f.detach())}
-        _progress(f"gradient with checkpoint interval {every}: {secs:.2f} s")
+        res[every] = {"seconds": round(secs, 3), "first_call_seconds": round(times[0], 3),
+                      "device_bytes": m.device_bytes(), "misfit": float(f.detach())}
+        _progress(f"gradient with checkpoint interval {every}: {secs:.2f} s "
+                  f"(first call {times[0]:.2f} s)")
         if every == a.c4_every:
             model = m
     bitwise = all(np.array_equal(x, y) for x, y in zip(grads[1], grads[a.c4_every]))

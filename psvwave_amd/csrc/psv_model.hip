@@ -943,8 +943,8 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
   if (cfg_.nt % cfg_.snapshot_interval != 0)
     throw ValueError("Snapshot interval and size of accumulator don't match!"
                      " (nt must be a multiple of snapshot_interval)");
-  if (!verbose && !graph_mode_ && (ckpt_every_ == 1 || !simulate_adjoint) &&
-      timing_stride_ <= 0 && cfg_.n_shots > 1) {
+  if (!verbose && (ckpt_every_ == 1 || !simulate_adjoint) && timing_stride_ <= 0 &&
+      cfg_.n_shots > 1) {
     run_model_two_lanes(simulate_adjoint);
     return;
   }
@@ -991,7 +991,13 @@ bool Model::ensure_defer_buffer() {
 // bitwise that of the sequential calls.  The host enqueues the lanes in
 // interleaved chunks of steps so that neither stream's queue runs dry.
 // (With checkpointing only the forward-only driver uses two lanes: the
-// checkpointed adjoint recomputes segments on the model stream.)
+// checkpointed adjoint recomputes segments on the model stream.)  In graph
+// mode each job -- one time loop with its prologue -- is captured once as a
+// HIP graph per (lane, shot, kind) and replayed, so the host issues one
+// launch per time loop.  Two lanes need two hardware queues: with more live
+// streams in the process than the runtime's queues (GPU_MAX_HW_QUEUES, 4 by
+// default) the lanes may share one and run one after the other -- the
+// results are the same, the overlap is lost.
 void Model::run_model_two_lanes(bool adj) {
   const int ns = cfg_.n_shots, nt = cfg_.nt;
   ensure_snapshots();
@@ -1090,13 +1096,12 @@ void Model::run_model_two_lanes(bool adj) {
         continue;
       }
       const int shot = jb.shot;
-      if (!ln.started) {
+      // everything a job enqueues before its time steps: the adjoint source
+      // of its shot, the zeroed deferred-term buffer, the zeroed state
+      auto prologue = [&]() {
         if (fwd) {
           ln.a = forward_args(shot);
         } else {
-          // the adjoint source of this shot once its forward has run
-          if (fwd_lane[shot] != l)
-            hip_check(hipStreamWaitEvent(ln.s, lane_events_[shot], 0), "hipStreamWaitEvent");
           const size_t off = (size_t)shot * cfg_.nr * nt;
           hip_check(launch_adjoint_sources(syn_[0] + off, syn_[1] + off, obs_[0] + off,
                                            obs_[1] + off, adj_[0] + off, adj_[1] + off,
@@ -1114,19 +1119,58 @@ void Model::run_model_two_lanes(bool adj) {
             hip_check(launch_fill(ln.st[p][f], npad_, 0.0, ln.s), "zero state");
         ln.k = 0;
         ln.parity = 0;
-        ln.started = true;
-      }
+      };
       SlotFn slot = [this, shot](int it, double **dst) {
         return stored_slot(shot, it, dst);
       };
-      const int k1 = std::min(ln.k + chunk, nt);
-      ln.parity = enqueue_range(ln.a, ln.st, ln.parity, fwd, ln.k, k1, true, slot, nullptr,
-                                false, ln.s, jb.kind == kAdjDeferred ? defer_ : nullptr);
-      ln.k = k1;
-      if (ln.k == nt) {
-        if (ln.a.rec_increments)
+      auto steps = [&](int k1) {
+        ln.parity = enqueue_range(ln.a, ln.st, ln.parity, fwd, ln.k, k1, true, slot, nullptr,
+                                  false, ln.s, jb.kind == kAdjDeferred ? defer_ : nullptr);
+        ln.k = k1;
+        if (ln.k == nt && ln.a.rec_increments)
           hip_check(launch_receiver_scan(ln.a.rtf_x, ln.a.rtf_z, cfg_.nr, nt, ln.s),
                     "receiver scan");
+      };
+      if (!ln.started && !fwd && fwd_lane[shot] != l)  // the shot's forward has run
+        hip_check(hipStreamWaitEvent(ln.s, lane_events_[shot], 0), "hipStreamWaitEvent");
+      if (graph_mode_) {
+        // the whole job as one captured graph (kept per lane, shot and kind)
+        const long long key = -1 - (((long long)l * ns + shot) * 4 + (long long)jb.kind);
+        auto gi = graphs_.find(key);
+        if (gi == graphs_.end()) {
+          hipGraph_t graph = nullptr;
+          hip_check(hipStreamBeginCapture(ln.s, hipStreamCaptureModeThreadLocal),
+                    "hipStreamBeginCapture");
+          const int64_t saved = launches_;
+          try {
+            prologue();
+            steps(nt);
+          } catch (...) {
+            hipGraph_t g2 = nullptr;
+            (void)hipStreamEndCapture(ln.s, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            launches_ = saved;
+            throw;
+          }
+          launches_ = saved;
+          hip_check(hipStreamEndCapture(ln.s, &graph), "hipStreamEndCapture");
+          hipGraphExec_t exec = nullptr;
+          hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          hip_check(e, "hipGraphInstantiate");
+          gi = graphs_.emplace(key, exec).first;
+        }
+        hip_check(hipGraphLaunch(gi->second, ln.s), "hipGraphLaunch");
+        launches_ += nt;
+        ln.k = nt;
+      } else {
+        if (!ln.started) {
+          prologue();
+          ln.started = true;
+        }
+        steps(std::min(ln.k + chunk, nt));
+      }
+      if (ln.k == nt) {
         if (fwd) {
           hip_check(hipEventRecord(lane_events_[shot], ln.s), "hipEventRecord");
           fwd_enqueued[shot] = 1;

@@ -122,3 +122,18 @@ def test_forward_lanes_with_checkpointing():
         m.map_kernels_to_velocity()
     for x, y in zip(a.get_kernels(), b.get_kernels()):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("variant,adjoint", [(9, True), (9, False), (6, True), (0, True)])
+def test_two_lane_graphs_are_bitwise_sequential(variant, adjoint):
+    """Graph mode: each time loop of the two-lane run_model is captured once
+    and replayed; the second call replays the cached graphs."""
+    cfg = _config(3)
+    a, b = _prepared(cfg, variant), _prepared(cfg, variant)
+    a.set_graph_mode(True)
+    _sequential(b, adjoint)
+    want = _outputs(b, adjoint)
+    for rep in range(2):
+        a.run_model(False, adjoint)
+        for k, (x, y) in enumerate(zip(_outputs(a, adjoint), want)):
+            assert np.array_equal(x, y), (variant, adjoint, rep, k)
