@@ -106,6 +106,9 @@ def _args():
     p.add_argument("--c5-along-path", type=float, default=0.5,
                    help="fraction of the training pairs taken part of the way "
                         "from the background to the true model")
+    p.add_argument("--c5-networks", type=int, default=2,
+                   help="networks trained (seeds 0, 1, ...) whose directions "
+                        "are averaged")
     p.add_argument("--c5-no-mirror", action="store_true",
                    help="use the network's prediction alone instead of its "
                         "mean with the mirrored prediction of the mirrored "
@@ -335,7 +338,8 @@ def learned_fwi(a):
     a held-out model and use the same line search.  `value` is the learned
     run's time to reach the misfit the plain run ends at, divided by the
     plain run's time (lower is better).  Training and inference run on the
-    library's bfloat16 MFMA convolution kernels (HipUNetTrainer, HipUNet)."""
+    library's bfloat16 MFMA convolution kernels (HipUNetTrainer, HipUNet);
+    the learned direction is the mean over --c5-networks trainings."""
     import torch
     import psvwave_amd
     from psvwave_amd import precond
@@ -344,13 +348,14 @@ def learned_fwi(a):
                                           along_path=a.c5_along_path)
     t_pairs = time.perf_counter() - t0
     _progress(f"{a.c5_pairs} training pairs in {t_pairs:.1f} s")
-    torch.manual_seed(0)
-    net = precond.make_unet()
+    unets, hist = [], []
     t0 = time.perf_counter()
-    hist = precond.train_hip(net, grads, perts, epochs=a.c5_epochs, log=_progress)
+    for seed in range(a.c5_networks):
+        torch.manual_seed(seed)
+        net = precond.make_unet()
+        hist.append(precond.train_hip(net, grads, perts, epochs=a.c5_epochs, log=_progress))
+        unets.append(precond.HipUNet(net.eval()))
     t_train = time.perf_counter() - t0
-    net.eval()
-    unet = precond.HipUNet(net)
     bg, true = precond.c5_models(np.random.default_rng(12345))
     model = psvwave_amd.fdModel(precond.c5_config())
     model.set_parameter_fields(*true)
@@ -361,8 +366,8 @@ def learned_fwi(a):
                              log=lambda m: _progress("plain " + m))
     target = h_plain[-1][1]
     h_learn, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
-                             direction=lambda g, f: precond.learned_direction_hip(
-                                 unet, g, f, mirror=not a.c5_no_mirror),
+                             direction=lambda g, f: precond.learned_direction_ensemble(
+                                 unets, g, f, mirror=not a.c5_no_mirror),
                              target=target, log=lambda m: _progress("learned " + m))
     t_plain = precond.time_to_misfit(h_plain, target)
     t_learn = precond.time_to_misfit(h_learn, target)
@@ -376,9 +381,10 @@ def learned_fwi(a):
         "config": {"workload": "C5 learned preconditioner", "grid": "316x188", "nt": 2500,
                    "shots": 6, "receivers": 48, "training_pairs": a.c5_pairs,
                    "pairs_along_path": a.c5_along_path, "mirror_average": not a.c5_no_mirror,
+                   "networks": a.c5_networks,
                    "epochs": a.c5_epochs, "fwi_iterations": a.c5_iterations},
         "pairs_seconds": round(t_pairs, 1), "train_seconds": round(t_train, 1),
-        "train_loss": [round(x, 5) for x in hist],
+        "train_loss": [[round(x, 5) for x in h] for h in hist],
         "plain": {"seconds": round(h_plain[-1][0], 2), "initial_misfit": h_plain[0][1],
                   "final_misfit": h_plain[-1][1], "iterations": len(h_plain) - 1},
         "learned": {"seconds_to_target": round(t_learn, 2) if t_learn is not None else None,

@@ -463,6 +463,13 @@ def learned_direction_hip(unet, g, fields, mirror=False):
     return [yy * f for yy, f in zip(y, fields)]
 
 
+def learned_direction_ensemble(unets, g, fields, mirror=True):
+    """Mean of learned_direction_hip over several trained networks (the
+    spread between trainings is large; the mean does not hinge on one)."""
+    ds = [learned_direction_hip(u, g, fields, mirror=mirror) for u in unets]
+    return [sum(d[k] for d in ds) / len(ds) for k in range(len(fields))]
+
+
 class HipUNetTrainer:
     """Training of a ``UNet`` with this library's bfloat16 MFMA kernels:
     forward convolutions keep their bfloat16 pre-activations, the backward

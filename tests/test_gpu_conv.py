@@ -315,3 +315,19 @@ def test_mirror_averaged_direction():
         assert np.array_equal(d[k], 0.5 * (y[k] + ym[k]) * fields[k])
     d0 = precond.learned_direction_hip(hip, g, fields)
     assert all(np.array_equal(a, b * f) for a, b, f in zip(d0, y, fields))
+
+
+def test_ensemble_direction_is_the_mean():
+    import torch
+    from psvwave_amd import precond
+    unets = []
+    for seed in (3, 4):
+        torch.manual_seed(seed)
+        unets.append(precond.HipUNet(precond.make_unet(base=8).cuda().eval()))
+    rng = np.random.default_rng(8)
+    g = [rng.standard_normal((40, 30)) for _ in range(3)]
+    fields = [1000 + rng.random((40, 30)) for _ in range(3)]
+    d = precond.learned_direction_ensemble(unets, g, fields, mirror=True)
+    d0, d1 = (precond.learned_direction_hip(u, g, fields, mirror=True) for u in unets)
+    for k in range(3):
+        assert np.array_equal(d[k], (d0[k] + d1[k]) / 2)
