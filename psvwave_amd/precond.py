@@ -465,7 +465,8 @@ def learned_direction_hip(unet, g, fields, mirror=False):
 
 def learned_direction_ensemble(unets, g, fields, mirror=True):
     """Mean of learned_direction_hip over several trained networks (the
-    spread between trainings is large; the mean does not hinge on one)."""
+    outcome varies widely between trainings; averaging does not always
+    help)."""
     ds = [learned_direction_hip(u, g, fields, mirror=mirror) for u in unets]
     return [sum(d[k] for d in ds) / len(ds) for k in range(len(fields))]
 
