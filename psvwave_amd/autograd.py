@@ -11,25 +11,50 @@ agrees with finite differences of the misfit to a few percent.
 With ``model.set_checkpointing(k)`` the forward keeps every k-th snapshot
 and the backward recomputes the rest (revolve-style, one level), with
 identical gradients.
+
+The backward uses the synthetics and snapshots the model holds.  If the
+model has run anything since this misfit's forward (another ``misfit``, a
+forward of other parameters), the backward restores the forward's
+parameters and recomputes them first; if the observed data changed, it
+raises ``RuntimeError``.
 """
 import numpy as np
 import torch
 
 
+def _host(t):
+    return t.detach().to("cpu", torch.float64).numpy()
+
+
 class _Misfit(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vp, vs, rho, model):
-        model.set_parameter_fields(*(t.detach().to("cpu", torch.float64).numpy()
-                                     for t in (vp, vs, rho)))
+        model.set_parameter_fields(*(_host(t) for t in (vp, vs, rho)))
         # forward of every shot with snapshots, then the misfit
         # (src/fdModel.cpp:1291-1297); two HIP streams, bitwise the per-shot calls
         model.run_model(False, False)
         ctx.model = model
+        # the state the backward needs -- parameters, synthetics, snapshots
+        # and observed data -- is identified by the model's change counters
+        ctx.generation = model._generation
+        ctx.obs_generation = model._obs_generation
+        ctx.save_for_backward(vp, vs, rho)
         return torch.tensor(model.misfit, dtype=torch.float64, device=vp.device)
 
     @staticmethod
     def backward(ctx, grad):
         m = ctx.model
+        if m._obs_generation != ctx.obs_generation:
+            raise RuntimeError("the model's observed data changed after this misfit "
+                               "was computed; its gradient is no longer defined")
+        if m._generation != ctx.generation:
+            # the model ran something else since the forward (a line-search
+            # trial, another misfit): restore this forward's parameters and
+            # recompute its synthetics and snapshots (deterministic, so
+            # bitwise those of the forward)
+            vp, vs, rho = ctx.saved_tensors
+            m.set_parameter_fields(*(_host(t) for t in (vp, vs, rho)))
+            m.run_model(False, False)
         m.calculate_l2_adjoint_sources()
         m.reset_kernels()
         for s in range(m.n_shots):

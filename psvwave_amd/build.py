@@ -39,21 +39,29 @@ def hipcc():
 
 
 def build(verbose=False):
-    """Compile every source and link the shared library (always rebuilds)."""
+    """Compile every source (in parallel) and link the shared library
+    (always rebuilds)."""
+    from concurrent.futures import ThreadPoolExecutor
     cc = hipcc()
     with tempfile.TemporaryDirectory(prefix="psv_build_") as tmp:
-        objs = []
+        cmds, objs = [], []
         for src in SOURCES:
             obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
             lang = (["-x", "hip"] if src.endswith(".hip") else
                     ["-x", "c++", "-D__HIP_PLATFORM_AMD__",
                      f"-I{os.path.join(ROCM, 'include')}"])
-            cmd = [cc, *COMMON, *DEVICE, *lang, "-c",
-                   os.path.join(CSRC, src), "-o", obj]
-            if verbose:
-                print(" ".join(cmd), flush=True)
-            subprocess.run(cmd, check=True)
+            cmds.append([cc, *COMMON, *DEVICE, *lang, "-c",
+                         os.path.join(CSRC, src), "-o", obj])
             objs.append(obj)
+        if verbose:
+            for cmd in cmds:
+                print(" ".join(cmd), flush=True)
+        jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0)
+                          or os.cpu_count() or 1))
+        with ThreadPoolExecutor(jobs) as pool:
+            for r in pool.map(lambda c: subprocess.run(c), cmds):
+                if r.returncode:
+                    raise subprocess.CalledProcessError(r.returncode, r.args)
         tmp_lib = LIB + ".tmp"
         cmd = [cc, *DEVICE, "-shared", "-fPIC", "-o", tmp_lib, *objs]
         if verbose:

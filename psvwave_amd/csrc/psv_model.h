@@ -149,6 +149,7 @@ class Model {
   double timed_ms_ = 0.0;
   int64_t timed_ = 0, launches_ = 0;
   std::vector<hipEvent_t> events_;
+  size_t event_cursor_ = 0;  // next free entry of events_ in a timed loop
   bool graph_mode_ = false;
   std::map<long long, hipGraphExec_t> graphs_;
   void clear_graphs();

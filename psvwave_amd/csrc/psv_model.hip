@@ -489,14 +489,15 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
   const bool two = step_is_two_step(variant_) && !dump;
   const int single = (two || dump) ? step_single_variant(variant_) : variant_;
   const int nt = cfg_.nt, si = cfg_.snapshot_interval;
-  size_t next_event = 0;
+  // timing events continue from event_cursor_, which the caller resets once
+  // per timed loop: the segments of a checkpointed adjoint share one loop
   auto event = [&]() {
-    if (next_event == events_.size()) {
+    if (event_cursor_ == events_.size()) {
       hipEvent_t e;
       hip_check(hipEventCreate(&e), "hipEventCreate");
       events_.push_back(e);
     }
-    return events_[next_event++];
+    return events_[event_cursor_++];
   };
   for (int k = k0; k < k1;) {
     const int pair = two && k + 1 < k1;
@@ -756,6 +757,7 @@ void Model::run_steps(StepArgs &a, int i_shot, bool forward, bool store,
     return;
   }
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+  event_cursor_ = 0;
   enqueue_loop(a, i_shot, forward, store, &pairs, dump);
   sync();
   for (auto &p : pairs) {
@@ -861,6 +863,7 @@ void Model::adjoint(int i_shot, bool verbose) {
     timed_loop([&] { enqueue_checkpointed_adjoint(a, i_shot, nullptr); });
   } else if (ckpt_every_ > 1) {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pairs;
+    event_cursor_ = 0;
     enqueue_checkpointed_adjoint(a, i_shot, &pairs);
     sync();
     for (auto &p : pairs) {

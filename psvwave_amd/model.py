@@ -101,6 +101,9 @@ class fdModel:
         info = _lib.Info()
         _lib.check(lib.psv_get_info(self._h, C.byref(info)))
         self._info = {n: getattr(info, n) for n, _ in _lib.Info._fields_}
+        # change counters: parameters, synthetics or snapshots; observed data
+        self._generation = 0
+        self._obs_generation = 0
         offs = np.zeros(self.n_shots + 1, np.int32)
         srcs = np.zeros(max(self.n_sources, 1), np.int32)
         _lib.check(lib.psv_get_which(self._h, _lib.iptr(offs), _lib.iptr(srcs)))
@@ -162,6 +165,7 @@ class fdModel:
         `omp_threads_override` is accepted for signature compatibility; the
         simulation runs on the GPU and does not use host threads.
         """
+        self._generation += 1
         _lib.check(self._lib.psv_forward(self._h, int(i_shot),
                                          int(bool(store_fields)),
                                          int(bool(verbose)),
@@ -194,6 +198,7 @@ class fdModel:
     def run_model(self, verbose=False, simulate_adjoint=True):
         """Forward all shots, misfit, and optionally adjoint sources, adjoint
         of all shots and kernel mapping (src/fdModel.cpp:1291-1308)."""
+        self._generation += 1
         _lib.check(self._lib.psv_run_model(self._h, int(bool(verbose)),
                                            int(bool(simulate_adjoint))))
         self._refresh_misfit()
@@ -215,6 +220,7 @@ class fdModel:
 
     def set_parameter_fields(self, vp, vs, rho):
         """Set (vp, vs, rho), each (nx, nz), and update the Lame fields."""
+        self._generation += 1
         shape = (self.nx, self.nz)
         a = [_f64(vp, shape, "_vp"), _f64(vs, shape, "_vs"),
              _f64(rho, shape, "_rho")]
@@ -240,6 +246,7 @@ class fdModel:
 
     def set_model_vector(self, m):
         """Update vp, vs, rho of the free region from a model vector."""
+        self._generation += 1
         v = np.ascontiguousarray(np.asarray(m, dtype=np.float64).reshape(-1))
         _lib.check(self._lib.psv_set_model_vector(self._h, _lib.dptr(v), v.size))
 
@@ -268,6 +275,7 @@ class fdModel:
         return tuple(a)
 
     def set_synthetic_data(self, ux, uz):
+        self._generation += 1
         shape = (self.n_shots, self.nr, self.nt)
         a = [_f64(ux, shape, "ux"), _f64(uz, shape, "uz")]
         _lib.check(self._lib.psv_set_synthetic_data(self._h, *map(_lib.dptr, a)))
@@ -279,8 +287,9 @@ class fdModel:
         return tuple(a)
 
     def set_observed_data(self, ux, uz):
+        self._obs_generation += 1
         shape = (self.n_shots, self.nr, self.nt)
-        a = [_f64(ux, shape, "ux"), _f64(uz, shape, "ux")]
+        a = [_f64(ux, shape, "ux"), _f64(uz, shape, "uz")]
         _lib.check(self._lib.psv_set_observed_data(self._h, *map(_lib.dptr, a)))
 
     def get_adjoint_sources(self):
@@ -351,9 +360,11 @@ class fdModel:
         _lib.check(self._lib.psv_write_kernels(self._h))
 
     def load_receivers(self, verbose=False):
+        self._obs_generation += 1
         _lib.check(self._lib.psv_load_receivers(self._h, int(bool(verbose))))
 
     def load_model(self, de_path, vp_path, vs_path, verbose=False):
+        self._generation += 1
         _lib.check(self._lib.psv_load_model(self._h, str(de_path).encode(),
                                             str(vp_path).encode(),
                                             str(vs_path).encode(),
@@ -384,6 +395,7 @@ class fdModel:
         """Keep every `every`-th forward snapshot only and recompute the
         rest during adjoint_simulate (bit-identical kernels, 1/every of the
         snapshot memory, one extra forward per adjoint)."""
+        self._generation += 1
         _lib.check(self._lib.psv_set_checkpointing(self._h, int(every)))
 
     def set_timing(self, stride):

@@ -13,9 +13,7 @@ the surface.  Models are a smooth Gaussian-random-field background plus a
   velocity kernels to a relative model update.  ``train_hip`` /
   ``HipUNetTrainer`` train it and ``HipUNet`` runs it on this library's
   bfloat16 MFMA kernels (psv_conv.hip: convolutions, input and weight
-  gradients, pooling) with fp32 accumulation and fp32 master weights;
-  ``train`` / ``learned_direction`` do the same through torch autocast, as
-  a reference point.
+  gradients, pooling) with fp32 accumulation and fp32 master weights.
 * ``fwi``: steepest-descent FWI with a backtracking line search on the
   plain-forward misfit; with a preconditioner, its update direction is
   tried first and the plain gradient is the fallback when the line search
@@ -227,57 +225,6 @@ def make_unet(base=32):
     return _unet_class()(base)
 
 
-def train(net, grads, perts, epochs=20, batch=16, lr=2e-3, device="cuda", log=None):
-    """Fit the network to map normalised gradients to perturbations
-    (mean-squared error on the kernel region), bfloat16 autocast."""
-    import torch
-    net = net.to(device)
-    mask = torch.tensor(kernel_mask(grads.shape[-2:]), dtype=torch.float32, device=device)
-    x_all = torch.tensor(normalise_gradient(grads), device=device)
-    y_all = torch.tensor(perts, device=device)
-    opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=1e-4)
-    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, T_max=epochs)
-    n = x_all.shape[0]
-    gen = torch.Generator(device="cpu").manual_seed(0)
-    history = []
-    for ep in range(epochs):
-        te = time.perf_counter()
-        perm = torch.randperm(n, generator=gen)
-        tot = 0.0
-        for i in range(0, n, batch):
-            idx = perm[i:i + batch].to(device)
-            x, y = x_all[idx], y_all[idx]
-            if torch.rand(1, generator=gen).item() < 0.5:  # mirror in x
-                x, y = torch.flip(x, dims=[2]), torch.flip(y, dims=[2])
-            with torch.autocast(device_type="cuda", dtype=torch.bfloat16,
-                                enabled=device.startswith("cuda")):
-                pred = net(x)
-            loss = (((pred.float() - y) * mask) ** 2).mean() / (mask.mean() * 1e-3)
-            opt.zero_grad(set_to_none=True)
-            loss.backward()
-            opt.step()
-            tot += float(loss.detach()) * x.shape[0]
-        sched.step()
-        history.append(tot / n)
-        if log:
-            log(f"epoch {ep + 1}/{epochs}: loss {history[-1]:.5f} "
-                f"({time.perf_counter() - te:.1f} s)")
-    return history
-
-
-def learned_direction(net, g, fields, device="cuda"):
-    """Model update direction from the network: the predicted relative
-    perturbation times the current parameters (descent direction up to the
-    line search's scale)."""
-    import torch
-    x = torch.tensor(normalise_gradient(np.stack(g))[None], device=device)
-    with torch.no_grad(), torch.autocast(device_type="cuda", dtype=torch.bfloat16,
-                                         enabled=device.startswith("cuda")):
-        y = net(x)
-    y = y.float()[0].cpu().numpy().astype(np.float64)
-    return [yy * f for yy, f in zip(y, fields)]
-
-
 def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,
         target=None, log=None, lcap=1.0, expand=False):
     """Steepest-descent FWI from `start` against model's observed data.
@@ -452,7 +399,9 @@ class HipUNet:
 
 
 def learned_direction_hip(unet, g, fields, mirror=False):
-    """As learned_direction, with the network run by HipUNet.  mirror: the
+    """Model update direction from the network run by HipUNet: the
+    predicted relative perturbation times the current parameters (a descent
+    direction up to the line search's scale).  mirror: the
     average of the prediction and the mirrored prediction for the mirrored
     gradient (the network is trained with mirror augmentation in x)."""
     x = normalise_gradient(np.stack(g))
@@ -701,7 +650,8 @@ class HipUNetTrainer:
         return xx
 
     def loss_and_grads(self, x, y, mask):
-        """Loss of train() and the parameter gradients (module, dW, db)."""
+        """Masked mean-squared-error loss and the parameter gradients (module,
+        dW, db)."""
         import torch
         L = self._layouts()
         nx, nz = x.shape[-2:]

@@ -182,3 +182,47 @@ def c4_models(cfg, seed=4):
     true = tuple(f * (1.0 + 0.1 * (gaussian_random_field(shape, 12, rng) - 0.5))
                  for f in start)
     return true, start
+
+
+def c1_config(nt=4000):
+    """Configuration of the small CPU-runnable workload: 200 x 100 inner
+    grid plus a 50-point taper (300 x 200), dx = dz = 1.25 m, dt = 0.25 ms,
+    nt = 4000, snapshot every 10 steps; two shots of one 50 Hz source each
+    (dip angles 90 and 180 degrees at inner (50, 90) and (150, 90)), 19
+    receivers at inner depth 10.  np_factor, the inner boundaries (10, 20),
+    the delay and the unused time shift are the reference test
+    configuration's values (tests/test_configurations/
+    default_testing_configuration.ini)."""
+    return {
+        "domain": dict(nt=nt, nx_inner=200, nz_inner=100, nx_inner_boundary=10,
+                       nz_inner_boundary=20, dx=1.25, dz=1.25, dt=2.5e-4),
+        "boundary": dict(np_boundary=50, np_factor=0.015),
+        "medium": dict(scalar_rho=1500.0, scalar_vp=2000.0, scalar_vs=800.0),
+        "sources": dict(peak_frequency=50.0, n_sources=2, n_shots=2,
+                        source_timeshift=0.005, delay_cycles_per_shot=24,
+                        moment_angles=[90.0, 180.0], ix_sources=[50, 150],
+                        iz_sources=[90, 90],
+                        which_source_to_fire_in_which_shot=[[0], [1]]),
+        "receivers": dict(nr=19, ix_receivers=list(range(10, 200, 10)),
+                          iz_receivers=[10] * 19),
+        "inversion": dict(snapshot_interval=10),
+        "basis": dict(npx=1, npz=1),
+        "output": dict(observed_data_folder=".", stf_folder="."),
+    }
+
+
+def c1_models(cfg):
+    """(true, start) models of the small workload on the full (nx, nz) grid:
+    the start model is homogeneous (vp 2000, vs 800, rho 1500); the true
+    model has vp and vs 10 % higher inside the circle of radius 15 grid
+    points about the grid centre, (ix - 149.5)^2 + (iz - 99.5)^2 < 15^2."""
+    d, npb = cfg["domain"], cfg["boundary"]["np_boundary"]
+    nx, nz = d["nx_inner"] + 2 * npb, d["nz_inner"] + 2 * npb
+    med = cfg["medium"]
+    start = (np.full((nx, nz), med["scalar_vp"]), np.full((nx, nz), med["scalar_vs"]),
+             np.full((nx, nz), med["scalar_rho"]))
+    ix = np.arange(nx, dtype=np.float64)[:, None]
+    iz = np.arange(nz, dtype=np.float64)[None, :]
+    circle = ((ix - (nx - 1) / 2) ** 2 + (iz - (nz - 1) / 2) ** 2) < 15.0 ** 2
+    true = (start[0] * (1 + 0.1 * circle), start[1] * (1 + 0.1 * circle), start[2].copy())
+    return true, start

@@ -91,3 +91,37 @@ def test_autograd_matches_kernels_and_finite_differences():
         fd = (float(misfit(*fp, m)) - float(misfit(*fm, m))) / (2 * eps)
         g = float((start[p].grad * d).sum())
         assert g == pytest.approx(fd, rel=0.10), (p, g, fd)
+
+
+def test_autograd_backward_after_another_forward():
+    """A second misfit() on the same model between forward and backward (a
+    line-search trial) must not change the first graph's gradient; changed
+    observed data make the backward raise."""
+    torch = pytest.importorskip("torch")
+    import psvwave_amd
+    from psvwave_amd.autograd import misfit
+    cfg = small_config(**{"domain.nt": 300})
+    m = psvwave_amd.fdModel(cfg)
+    vp0, vs0, rho0 = m.get_parameter_fields()
+    rng = np.random.default_rng(8)
+    m.set_parameter_fields(vp0 * (1 + 0.02 * rng.random(vp0.shape)), vs0, rho0)
+    for s in range(m.n_shots):
+        m.forward_simulate(s, store_fields=False)
+    m.set_observed_data(*m.get_synthetic_data())
+    a = [torch.tensor(x, requires_grad=True) for x in (vp0, vs0, rho0)]
+    fa = misfit(*a, m)
+    fa.backward()
+    want = [t.grad.clone() for t in a]
+    b = [torch.tensor(x, requires_grad=True) for x in (vp0, vs0, rho0)]
+    fb = misfit(*b, m)
+    c = [torch.tensor(x * 1.01) for x in (vp0, vs0, rho0)]
+    fc = misfit(*c, m)  # the model now holds the state at c
+    assert float(fc) != float(fb)
+    fb.backward()
+    for g, w in zip(b, want):
+        assert torch.equal(g.grad, w)
+    d = [torch.tensor(x, requires_grad=True) for x in (vp0, vs0, rho0)]
+    fd = misfit(*d, m)
+    m.set_observed_data(*(o * 1.5 for o in m.get_observed_data()))
+    with pytest.raises(RuntimeError, match="observed data"):
+        fd.backward()

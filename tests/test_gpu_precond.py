@@ -9,6 +9,18 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
+def _torch_direction(net, g, fields):
+    """The same update direction with the network run by torch's bfloat16
+    autocast (comparison only; the library's path is HipUNet)."""
+    import torch
+    from psvwave_amd import precond
+    x = torch.tensor(precond.normalise_gradient(np.stack(g))[None], device="cuda")
+    with torch.no_grad(), torch.autocast(device_type="cuda", dtype=torch.bfloat16):
+        y = net.to("cuda")(x)
+    y = y.float()[0].cpu().numpy().astype(np.float64)
+    return [yy * f for yy, f in zip(y, fields)]
+
+
 def test_precond_pipeline_small():
     torch = pytest.importorskip("torch")
     import psvwave_amd
@@ -41,7 +53,7 @@ def test_precond_pipeline_small():
         m.forward_simulate(s, store_fields=False)
     m.set_observed_data(*m.get_synthetic_data())
     for direction in (None, lambda gg, f: precond.learned_direction_hip(unet, gg, f),
-                      lambda gg, f: precond.learned_direction(net, gg, f)):
+                      lambda gg, f: _torch_direction(net, gg, f)):
         h, fields = precond.fwi(m, bg, iterations=3, direction=direction)
         misfits = [x[1] for x in h]
         assert all(b <= a for a, b in zip(misfits, misfits[1:]))

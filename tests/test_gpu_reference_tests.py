@@ -2,7 +2,9 @@
 
 tests/test_file_constructor.cpp, test_variable_constructor.cpp,
 test_constructor_comparison.cpp, test_copy_constructor.cpp (bitwise-equal
-sums of rtf_ux), tests/python/test_copy.py and test_descent.py.
+sums of rtf_ux), tests/python/test_copy.py and test_descent.py.  The
+Python tests import the reference's module name, ``psvWave``, as the
+reference's do.
 """
 import numpy as np
 import pytest
@@ -64,15 +66,15 @@ def test_copy_constructor():
 
 
 def test_copy():
-    import psvwave_amd
-    model = psvwave_amd.fdModel(DEFAULT_INI)
+    import psvWave
+    model = psvWave.fdModel(DEFAULT_INI)
     model2 = model.copy()
     assert model2.nx == model.nx and model2.free_parameters == model.free_parameters
 
 
 def test_copy_modify():
-    import psvwave_amd
-    model1 = psvwave_amd.fdModel(DEFAULT_INI)
+    import psvWave
+    model1 = psvWave.fdModel(DEFAULT_INI)
     model2 = model1.copy()
     model1.set_model_vector(model1.get_model_vector() + 1)
     model2.set_model_vector(model2.get_model_vector() - 1)
@@ -82,8 +84,8 @@ def test_copy_modify():
 def test_descent():
     """tests/python/test_descent.py: a -10% vp/vs circle, 3 steps of
     m -= 0.25 g must lower the misfit."""
-    import psvwave_amd
-    model = psvwave_amd.fdModel(DEFAULT_INI)
+    import psvWave
+    model = psvWave.fdModel(DEFAULT_INI)
     IX, IZ = model.get_coordinates(True)
     vp, vs, rho = model.get_parameter_fields()
     vp_starting, vs_starting, rho_starting = vp, vs, rho

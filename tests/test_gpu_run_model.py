@@ -89,7 +89,8 @@ def test_run_model_timing_and_fallbacks():
     assert timed == 2 * 3 * cfg["domain"]["nt"] and ms > 0
     for x, y in zip(_outputs(m, True), want):
         assert np.array_equal(x, y)
-    # per-launch timing, graph mode and checkpointing take the sequential path
+    # per-launch timing and checkpointing take the sequential path; graph
+    # mode keeps the two lanes with one captured graph per time loop
     for setup in (lambda mm: mm.set_timing(7), lambda mm: mm.set_graph_mode(True),
                   lambda mm: mm.set_checkpointing(4)):
         mm = _prepared(cfg, 9)

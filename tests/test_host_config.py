@@ -182,3 +182,14 @@ def test_list_parsing_edge_cases(tmp_path):
     with pytest.raises(ValueError, match="closing"):
         psvwave_amd.Config(_write(tmp_path, INI.format(nt=100, which="{{0,2},{1}}")
                                   .replace("iz_sources = {5,5,5}", "iz_sources = 5,5,5")))
+
+
+def test_reference_module_name():
+    """``import psvWave`` (the reference's module, psvWave/__init__.py:6)
+    gives this build's class and INI helpers."""
+    import psvWave
+    import psvwave_amd
+    assert psvWave.fdModel is psvwave_amd.fdModel
+    assert psvWave.get_dictionary() == psvwave_amd.get_dictionary()
+    assert psvWave.write_dictionary is psvwave_amd.write_dictionary
+    assert psvWave.from_dictionary is psvwave_amd.from_dictionary
