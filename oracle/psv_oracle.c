@@ -495,6 +495,30 @@ static void correlate(oracle_model *m, int i_shot, int it) {
   }
 }
 
+/* Test hooks: write one stored snapshot (allocating the store if needed)
+ * and apply the correlation of time step `it` with the current state. */
+int oracle_set_snapshot(oracle_model *m, int i_shot, int i_snap, const double *vx,
+                        const double *vz, const double *txx, const double *tzz,
+                        const double *txz) {
+  if (i_shot < 0 || i_shot >= m->n_shots || i_snap < 0 || i_snap >= m->snapshots)
+    return -1;
+  const long n = (long)m->nx * m->nz;
+  if (!m->accu[0])
+    for (int k = 0; k < 5; ++k)
+      m->accu[k] = calloc((size_t)m->n_shots * m->snapshots * n, sizeof(double));
+  const long off = ((long)i_shot * m->snapshots + i_snap) * n;
+  const double *ins[5] = {vx, vz, txx, tzz, txz};
+  for (int k = 0; k < 5; ++k) memcpy(m->accu[k] + off, ins[k], n * sizeof(double));
+  return 0;
+}
+int oracle_correlate_step(oracle_model *m, int i_shot, int it) {
+  if (!m->accu[0] || i_shot < 0 || i_shot >= m->n_shots || it < 0 || it >= m->nt ||
+      it % m->si != 0)
+    return -1;
+  correlate(m, i_shot, it);
+  return 0;
+}
+
 /* src/fdModel.cpp:789-969 */
 int oracle_adjoint(oracle_model *m, int i_shot) {
   if (check_shot(m, i_shot)) return -1;

@@ -90,6 +90,10 @@ void oracle_set_state(oracle_model *m, const double *vx, const double *vz,
                       const double *txx, const double *tzz, const double *txz);
 void oracle_stress_phase(oracle_model *m, int adjoint);
 void oracle_velocity_phase(oracle_model *m, int adjoint);
+int oracle_set_snapshot(oracle_model *m, int i_shot, int i_snap, const double *vx,
+                        const double *vz, const double *txx, const double *tzz,
+                        const double *txz);
+int oracle_correlate_step(oracle_model *m, int i_shot, int it);
 int oracle_get_snapshot(const oracle_model *m, int i_shot, int i_snap,
                         double *vx, double *vz, double *txx, double *tzz,
                         double *txz);

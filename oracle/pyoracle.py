@@ -47,6 +47,8 @@ def _load():
     lib.oracle_l2_misfit.restype = C.c_double
     lib.oracle_l2_misfit.argtypes = [P]
     lib.oracle_get_snapshot.argtypes = [P, C.c_int, C.c_int] + [dp] * 5
+    lib.oracle_set_snapshot.argtypes = [P, C.c_int, C.c_int] + [dp] * 5
+    lib.oracle_correlate_step.argtypes = [P, C.c_int, C.c_int]
     for name in ["oracle_l2_adjoint_sources", "oracle_reset_kernels",
                  "oracle_map_kernels"]:
         getattr(lib, name).argtypes = [P]
@@ -272,6 +274,19 @@ class OracleModel:
 
     def velocity_phase(self, adjoint=False):
         self._lib.oracle_velocity_phase(self._h, int(adjoint))
+
+    def set_snapshot(self, i_shot, i_snap, vx, vz, txx, tzz, txz):
+        arrs = [_f64(a) for a in (vx, vz, txx, tzz, txz)]
+        if self._lib.oracle_set_snapshot(self._h, int(i_shot), int(i_snap),
+                                         *map(_dptr, arrs)):
+            raise ValueError("no such snapshot")
+
+    def correlate_step(self, i_shot, it):
+        """The kernel correlation of adjoint time step `it` (a multiple of
+        the snapshot interval) between the current state and the stored
+        snapshot of that step."""
+        if self._lib.oracle_correlate_step(self._h, int(i_shot), int(it)):
+            raise ValueError("no stored snapshot for this step")
 
     def get_snapshot(self, i_shot, i_snap):
         a = self._grid(5)

@@ -9,8 +9,10 @@ independently of any retyping of its formulas.
   each of the five updates, forward and adjoint, with distinct lm / la / mu
   and dx != dz, so a wrong coefficient, offset, sign or spacing fails.
 * Physics: the gradient matches central finite differences of the misfit
-  per parameter block, and the velocity-kernel map is the chain rule of
-  lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2.
+  per parameter block, the velocity-kernel map is the chain rule of
+  lambda = rho (vp^2 - 2 vs^2), mu = rho vs^2, and one correlation step is
+  the textbook strain (and velocity) correlation, strains solved from the
+  stresses by Hooke's law.
 """
 import json
 import math
@@ -350,6 +352,63 @@ def test_velocity_kernel_map_is_the_chain_rule(oracle_lib):
     got = _map_with_oracle(oracle_lib, cfg, (vp, vs, rho), (lam_k, mu_k, rho_k))
     for g, e in zip(got, expect):
         np.testing.assert_allclose(g, e, rtol=1e-6, atol=1e-9 * np.abs(e).max())
+
+
+def test_correlation_is_the_strain_correlation(oracle_lib):
+    """One correlation step of the adjoint (forward snapshot x adjoint state)
+    against the textbook sensitivity densities: strains from stresses by
+    solving the plane-strain Hooke system [[l + 2m, l], [l, l + 2m]] e = s
+    (numpy), e_xz = s_xz / (2 m); then
+        K_lambda += w tr(e) tr(e'),   K_mu += w 2 e : e',
+        K_rho    -= w v . v'
+    with w = snapshot_interval * dt, on the kernel region only.  This pins the
+    lambda, mu and rho correlations point by point, and with the chain-rule
+    pin of the velocity map the vp and rho gradient blocks."""
+    cfg = _tiny(nx_inner=16, nz_inner=12, nt=3, **{"inversion.snapshot_interval": 3,
+                                                    "domain.nx_inner_boundary": 2,
+                                                    "domain.nz_inner_boundary": 3})
+    o = oracle_lib.OracleModel(cfg)
+    rng = np.random.default_rng(11)
+    shape = (o.nx, o.nz)
+    vp = 2.5 + rng.random(shape)
+    vs = 0.8 + 0.4 * rng.random(shape)
+    rho = 1.5 + rng.random(shape)
+    o.set_parameter_fields(vp, vs, rho)
+    fwd = [rng.normal(size=shape) for _ in range(5)]
+    adj = [rng.normal(size=shape) for _ in range(5)]
+    o.set_snapshot(0, 0, *fwd)
+    o.set_state(*adj)
+    o.reset_kernels()
+    o.correlate_step(0, 0)
+    lam_k, mu_k, rho_k = o.get_lame_kernels()
+
+    lam = rho * (vp ** 2 - 2 * vs ** 2)
+    mu = rho * vs ** 2
+    hooke = np.empty(shape + (2, 2))
+    hooke[..., 0, 0] = hooke[..., 1, 1] = lam + 2 * mu
+    hooke[..., 0, 1] = hooke[..., 1, 0] = lam
+
+    def strains(txx, tzz, txz):
+        e = np.linalg.solve(hooke, np.stack([txx, tzz], axis=-1)[..., None])[..., 0]
+        return e[..., 0], e[..., 1], txz / (2 * mu)
+
+    exx, ezz, exz = strains(*fwd[2:])
+    axx, azz, axz = strains(*adj[2:])
+    w = 3 * cfg["domain"]["dt"]
+    region = np.zeros(shape, bool)
+    d = cfg["domain"]
+    region[d["nx_inner_boundary"]:d["nx_inner"] - d["nx_inner_boundary"],
+           d["nz_inner_boundary"]:d["nz_inner"] - d["nz_inner_boundary"]] = True
+    expect = {
+        "lambda": w * (exx + ezz) * (axx + azz),
+        "mu": w * 2 * (exx * axx + ezz * azz + 2 * exz * axz),
+        "rho": -w * (fwd[0] * adj[0] + fwd[1] * adj[1]),
+    }
+    for name, got in (("lambda", lam_k), ("mu", mu_k), ("rho", rho_k)):
+        e = np.where(region, expect[name], 0.0)
+        np.testing.assert_allclose(got, e, rtol=1e-9, atol=1e-13 * np.abs(e).max(),
+                                   err_msg=name)
+        assert np.all(got[~region] == 0), name
 
 
 def _map_with_oracle(oracle_lib, cfg, fields, kernels):
