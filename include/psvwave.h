@@ -213,6 +213,23 @@ int psv_get_step_timing(psv_model *m, double *total_ms, int64_t *timed,
 int psv_reset_step_timing(psv_model *m);
 /* Device bytes held by the model. */
 int psv_device_bytes(const psv_model *m, int64_t *bytes);
+/* L2 misfit of device-resident traces without any allocation (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same reduction as psv_calculate_l2_misfit, whose
+ * result it reproduces bit for bit: *misfit (a device pointer) =
+ * sum over i < n of 0.5 dt ((obs_ux[i] - syn_ux[i])^2 + (obs_uz[i] -
+ * syn_uz[i])^2), the four arrays device buffers of n doubles each (a
+ * model's traces are n = n_shots * nr * nt values in [shot][receiver][it]
+ * order).  Enqueued on `stream` (a hipStream_t; NULL = the default stream),
+ * not synchronised.  Its 512 compensated partial sums live in a fixed 8 KiB
+ * device array of the library, one per GPU, held until the process ends;
+ * every call overwrites them, so calls that overlap on one GPU race.
+ * n outside [0, PSV_NOALLOC_MAX_TERMS] is rejected with PSV_EVALUE. */
+#define PSV_NOALLOC_MAX_TERMS ((int64_t)1 << 40)
+int psv_l2_misfit_noalloc(const double *syn_ux, const double *syn_uz,
+                          const double *obs_ux, const double *obs_uz, int64_t n,
+                          double dt, double *misfit, void *stream);
 /* Streaming copy of `bytes` (a multiple of 16) device-to-device with a plain
  * copy kernel, `iters` times; *ms is the mean kernel time (HIP events).
  * Reference point for byte counters and sustainable bandwidth. */

@@ -6,6 +6,7 @@
 
 #include <cstring>
 #include <new>
+#include <stdexcept>
 #include <string>
 
 #include "psv_batch.h"
@@ -339,6 +340,28 @@ int psv_calculate_l2_misfit(psv_model *m, double *misfit) {
   return guard([&] {
     double v = M(m).calculate_l2_misfit();
     if (misfit) *misfit = v;
+  });
+}
+
+int psv_l2_misfit_noalloc(const double *syn_ux, const double *syn_uz,
+                          const double *obs_ux, const double *obs_uz, int64_t n,
+                          double dt, double *misfit, void *stream) {
+  return guard([&] {
+    if (n < 0 || n > PSV_NOALLOC_MAX_TERMS)
+      throw psv::ValueError("psv_l2_misfit_noalloc: n = " + std::to_string(n) +
+                            " terms per component is outside [0, " +
+                            std::to_string((long long)PSV_NOALLOC_MAX_TERMS) + "]");
+    need(misfit, "misfit");
+    if (n > 0) {
+      need(syn_ux, "syn_ux");
+      need(syn_uz, "syn_uz");
+      need(obs_ux, "obs_ux");
+      need(obs_uz, "obs_uz");
+    }
+    hipError_t e = psv::launch_misfit_noalloc(syn_ux, syn_uz, obs_ux, obs_uz, n, 0.5 * dt,
+                                              misfit, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess)
+      throw std::runtime_error(std::string("psv_l2_misfit_noalloc: ") + hipGetErrorString(e));
   });
 }
 

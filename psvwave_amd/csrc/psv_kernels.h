@@ -305,6 +305,11 @@ constexpr int kMisfitBlocks = 512;
 hipError_t launch_misfit(const double *sx, const double *sz, const double *ox,
                          const double *oz, long long n, double half_dt,
                          double *partials, double *out, hipStream_t s);
+// The same reduction with its partial sums in a fixed device array of the
+// library (no allocation; 2 * kMisfitBlocks doubles per device, whatever n).
+hipError_t launch_misfit_noalloc(const double *sx, const double *sz, const double *ox,
+                                 const double *oz, long long n, double half_dt,
+                                 double *out, hipStream_t s);
 hipError_t launch_model_vector(int mode, double *vp, double *vs, double *rho,
                                double *vec, int nx, int nxf, int nzf, int npx,
                                int npz, int gx0, int gz0, hipStream_t s);

@@ -23,6 +23,7 @@
 #include <hip/hip_ext.h>
 
 #include <cmath>
+#include <mutex>
 
 namespace psv {
 
@@ -2205,6 +2206,47 @@ hipError_t launch_misfit(const double *sx, const double *sz, const double *ox,
   hipLaunchKernelGGL(misfit_final_kernel, dim3(1), dim3(kBlock), 0, s,
                      partials, kMisfitBlocks, out);
   return hipGetLastError();
+}
+
+// Partial sums of the allocation-free misfit (launch_misfit_noalloc): the
+// same kMisfitBlocks compensated pairs that launch_misfit keeps in a model's
+// own buffer, here in a fixed array that every GPU running the library holds
+// once until the process ends.  Each call overwrites every pair before the
+// final kernel reads them, so nothing has to be cleared; calls that overlap
+// on one device race on it.
+static __device__ __align__(256) double scratch[2 * kMisfitBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is only a stand-in for the symbol).
+hipError_t scratch_on_current_device(double **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_misfit_noalloc(const double *sx, const double *sz, const double *ox,
+                                 const double *oz, long long n, double half_dt,
+                                 double *out, hipStream_t s) {
+  double *partials = nullptr;
+  hipError_t e = scratch_on_current_device(&partials);
+  if (e != hipSuccess) return e;
+  return launch_misfit(sx, sz, ox, oz, n, half_dt, partials, out, s);
 }
 
 hipError_t launch_model_vector(int mode, double *vp, double *vs, double *rho,
