@@ -10,12 +10,14 @@ with kernel correlation, kernel mapping -- 7.776e10 point-updates.
 Prints one JSON line.  ``value`` is point-updates per second of the whole
 step with all inputs resident in HBM.  ``roofline`` prices the step kernel
 (the dominant kernel) at its algorithmic bytes per launch over its average
-time per launch: HIP events on the model's stream bracket every forward and
-adjoint time loop (the step launches and the gaps between them), divided by
-the time steps launched.  ``roofline.traffic`` is the HBM bytes per launch
+duration when it runs alone (HIP start/end events of every launch of one
+shot's forward + adjoint on one stream, after the timed region);
+``roofline.device_aggregate`` prices the timed region instead: HIP events
+around every run_model (two time loops at a time on two streams), divided
+by the time steps launched.  ``roofline.traffic`` is the bytes per launch
 from the committed rocprofv3 counter summary for the kernel.
-``cpu_baseline`` times the CPU oracle (OpenMP, all host cores up to 16) on a
-bounded sample of the same grid.
+``cpu_baseline`` times the CPU oracle (OpenMP over the CPUs this process may
+use, threads bound to cores) on a bounded sample of the same grid.
 """
 import argparse
 import gc
@@ -67,7 +69,9 @@ def _args():
                         "6, 7 two steps per launch (64x8, 64x16); 8-13 lean "
                         "one-step kernels with 64 x 16, 8, 12, 20, 24, 32 "
                         "tiles (9 is the library default); 14 lean two "
-                        "steps per launch (64x8)")
+                        "steps per launch (64x8); 15-17 lean with 2, 3, 4 "
+                        "tiles of a strip per block; 18 lean 64x4; 19, 20 "
+                        "lean 64x8 / 64x12 at 6 / 5 waves per SIMD")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
@@ -100,15 +104,19 @@ def _args():
                         "unet: the preconditioner network's training step and "
                         "inference on the library's MFMA kernels vs torch")
     p.add_argument("--c4-directions", type=int, default=20)
-    p.add_argument("--c5-pairs", type=int, default=2048)
-    p.add_argument("--c5-epochs", type=int, default=20)
+    p.add_argument("--c5-pairs", type=int, default=10240)
+    p.add_argument("--c5-epochs", type=int, default=12)
     p.add_argument("--c5-iterations", type=int, default=30)
     p.add_argument("--c5-along-path", type=float, default=0.5,
                    help="fraction of the training pairs taken part of the way "
                         "from the background to the true model")
-    p.add_argument("--c5-networks", type=int, default=2,
-                   help="networks trained (seeds 0, 1, ...) whose directions "
-                        "are averaged")
+    p.add_argument("--c5-seeds", type=int, default=4,
+                   help="networks trained (seeds 0, 1, ...); each one's "
+                        "time-to-misfit is reported, with the median")
+    p.add_argument("--c5-holdout", type=int, default=2,
+                   help="held-out models the FWI runs on")
+    p.add_argument("--c5-widths", default="32,48,64,128",
+                   help="U-Net channel widths of the four levels")
     p.add_argument("--c5-no-mirror", action="store_true",
                    help="use the network's prediction alone instead of its "
                         "mean with the mirrored prediction of the mirrored "
@@ -129,16 +137,46 @@ def calibrate_copy(nbytes, iters=5):
     return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
 
+def _host_cpu():
+    """CPU model, sockets and physical cores of the host (lscpu), and the
+    logical CPUs this process may run on."""
+    import subprocess
+    info = {"model": None, "sockets": None, "physical_cores": None,
+            "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {k.strip(): v.strip() for k, _, v in
+              (line.partition(":") for line in out.splitlines())}
+        info["model"] = kv.get("Model name")
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        per = int(kv.get("Core(s) per socket", "0") or 0)
+        info["sockets"] = sockets or None
+        info["physical_cores"] = sockets * per or None
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    return info
+
+
 def cpu_baseline(nt_sample):
     """CPU oracle on a bounded sample: one shot, forward + adjoint, the
-    benchmark grid, nt_sample steps."""
+    benchmark grid, nt_sample steps.  The oracle runs one OpenMP thread per
+    CPU this process may use (its affinity mask), or OMP_NUM_THREADS if that
+    is set lower (a shared host's per-job CPU share), with threads bound
+    close, one place per core; the library reads the binding at load, so it
+    is set before the oracle is loaded."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle.pyoracle import OracleModel
     from psvwave_amd.synthetic import layered_faulted_config, layered_faulted_model
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
+    host = _host_cpu()
+    threads = host["affinity_cpus"] or 1
+    env_cap = os.environ.get("OMP_NUM_THREADS", "")
+    if env_cap.isdigit() and int(env_cap) > 0:
+        threads = min(threads, int(env_cap))
     OracleModel.set_threads(threads)
     cfg = layered_faulted_config(nt=nt_sample, n_shots=1)
     true = layered_faulted_model(cfg, faulted=False)
@@ -161,7 +199,25 @@ def cpu_baseline(nt_sample):
             "cores": threads, "kind": "port",
             "sample": (f"1 shot fwd+adj, {o.nx}x{o.nz} grid, nt={nt_sample} "
                        f"({points:.3e} point-updates, {secs:.2f} s, "
-                       f"OpenMP {threads} threads)")}
+                       f"OpenMP {threads} threads, OMP_PROC_BIND="
+                       f"{os.environ.get('OMP_PROC_BIND')}, OMP_PLACES="
+                       f"{os.environ.get('OMP_PLACES')})"),
+            "host": host, "omp_num_threads_env": env_cap or None}
+
+
+def kernel_alone_us(model):
+    """Average duration in microseconds of a step launch on its own: kernel
+    start/end events of every launch of one shot's forward (with snapshots)
+    and adjoint (with correlation), called shot by shot on one stream, so
+    the launches sample ordinary, snapshot and correlation steps in the
+    benchmark's proportions."""
+    model.set_timing(1)
+    model.reset_step_timing()
+    model.forward_simulate(0)
+    model.adjoint_simulate(0)
+    ms, timed, _ = model.step_timing()
+    model.set_timing(0)
+    return ms / max(timed, 1) * 1e3
 
 
 def monte_carlo(a):
@@ -332,71 +388,94 @@ def gradient_check(a):
 
 def learned_fwi(a):
     """C5: FWI time-to-misfit with the learned preconditioner against plain
-    steepest descent on the same GPU.  Training pairs (gradient at a smooth
-    background, true perturbation) come from the device solver; the U-Net is
-    trained in bfloat16 autocast; both FWI runs start from the background of
-    a held-out model and use the same line search.  `value` is the learned
-    run's time to reach the misfit the plain run ends at, divided by the
-    plain run's time (lower is better).  Training and inference run on the
-    library's bfloat16 MFMA convolution kernels (HipUNetTrainer, HipUNet);
-    the learned direction is the mean over --c5-networks trainings."""
+    steepest descent on the same GPU.  Training pairs (velocity kernels at a
+    background or part of the way to the true model, remaining relative
+    perturbation) come from the batched device solver; --c5-seeds U-Nets
+    are trained on the library's bfloat16 MFMA kernels (HipUNetTrainer), one
+    per seed; on each held-out model the plain run sets the target misfit
+    (its final one) and each network's learned run (its direction tried
+    first, the plain gradient as fallback, the same line search) is timed to
+    that misfit.  `value` is the median over seeds and held-out models of
+    learned time / plain time (lower is better); runs that never reach the
+    target count as infinite."""
     import torch
     import psvwave_amd
     from psvwave_amd import precond
+    widths = tuple(int(c) for c in a.c5_widths.split(","))
     t0 = time.perf_counter()
-    grads, perts = precond.gradient_pairs(a.c5_pairs, seed=7, workers=8, log=_progress,
+    grads, perts = precond.gradient_pairs(a.c5_pairs, seed=7, log=_progress,
                                           along_path=a.c5_along_path)
     t_pairs = time.perf_counter() - t0
     _progress(f"{a.c5_pairs} training pairs in {t_pairs:.1f} s")
-    unets, hist = [], []
-    t0 = time.perf_counter()
-    for seed in range(a.c5_networks):
+    unets, hist, t_train = [], [], []
+    for seed in range(a.c5_seeds):
+        t0 = time.perf_counter()
         torch.manual_seed(seed)
-        net = precond.make_unet()
+        net = precond.make_unet(widths)
         hist.append(precond.train_hip(net, grads, perts, epochs=a.c5_epochs, log=_progress))
         unets.append(precond.HipUNet(net.eval()))
-    t_train = time.perf_counter() - t0
-    bg, true = precond.c5_models(np.random.default_rng(12345))
+        t_train.append(time.perf_counter() - t0)
+        _progress(f"network {seed}: trained in {t_train[-1]:.1f} s")
+    params = sum(p.numel() for p in net.parameters())
+    del grads, perts
     model = psvwave_amd.fdModel(precond.c5_config())
-    model.set_parameter_fields(*true)
-    for s in range(model.n_shots):
-        model.forward_simulate(s, store_fields=False)
-    model.set_observed_data(*model.get_synthetic_data())
-    h_plain, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
-                             log=lambda m: _progress("plain " + m))
-    target = h_plain[-1][1]
-    h_learn, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
-                             direction=lambda g, f: precond.learned_direction_ensemble(
-                                 unets, g, f, mirror=not a.c5_no_mirror),
-                             target=target, log=lambda m: _progress("learned " + m))
-    t_plain = precond.time_to_misfit(h_plain, target)
-    t_learn = precond.time_to_misfit(h_learn, target)
+    runs = []
+    for h in range(a.c5_holdout):
+        bg, true = precond.c5_models(np.random.default_rng(12345 + h))
+        model.set_parameter_fields(*true)
+        for s in range(model.n_shots):
+            model.forward_simulate(s, store_fields=False)
+        model.set_observed_data(*model.get_synthetic_data())
+        h_plain, _ = precond.fwi(model, bg, iterations=a.c5_iterations,
+                                 log=lambda m: _progress(f"model {h} plain " + m))
+        target = h_plain[-1][1]
+        t_plain = precond.time_to_misfit(h_plain, target)
+        for seed, unet in enumerate(unets):
+            h_learn, _ = precond.fwi(
+                model, bg, iterations=a.c5_iterations,
+                direction=lambda g, f, u=unet: precond.learned_direction_hip(
+                    u, g, f, mirror=not a.c5_no_mirror),
+                target=target, log=lambda m: _progress(f"model {h} seed {seed} " + m))
+            t_learn = precond.time_to_misfit(h_learn, target)
+            runs.append({
+                "holdout": h, "seed": seed,
+                "ratio": round(t_learn / t_plain, 4) if t_learn is not None else None,
+                "plain_seconds": round(h_plain[-1][0], 2),
+                "learned_seconds_to_target": round(t_learn, 2) if t_learn is not None else None,
+                "learned_iterations": len(h_learn) - 1,
+                "fallbacks": int(sum(1 for x in h_learn if x[2])),
+                "initial_misfit": h_plain[0][1], "target_misfit": target})
+            _progress(f"model {h} seed {seed}: ratio {runs[-1]['ratio']}")
+    ratios = [r["ratio"] if r["ratio"] is not None else float("inf") for r in runs]
+    med = float(np.median(ratios))
     return {
         "metric": "FWI time-to-misfit, learned vs plain",
-        "value": round(t_learn / t_plain, 4) if t_learn is not None else None,
-        "unit": "learned time / plain time to the plain run's final misfit",
+        "value": round(med, 4) if np.isfinite(med) else None,
+        "unit": "learned time / plain time to the plain run's final misfit "
+                "(median over seeds and held-out models)",
         "n_gpus": 1, "higher_is_better": False,
         "dtype": "f64 solver, bf16 network (hand-written MFMA kernels)",
         "data": "synthetic (seeded GRF background + perturbation)",
         "config": {"workload": "C5 learned preconditioner", "grid": "316x188", "nt": 2500,
                    "shots": 6, "receivers": 48, "training_pairs": a.c5_pairs,
                    "pairs_along_path": a.c5_along_path, "mirror_average": not a.c5_no_mirror,
-                   "networks": a.c5_networks,
-                   "epochs": a.c5_epochs, "fwi_iterations": a.c5_iterations},
-        "pairs_seconds": round(t_pairs, 1), "train_seconds": round(t_train, 1),
-        "train_loss": [[round(x, 5) for x in h] for h in hist],
-        "plain": {"seconds": round(h_plain[-1][0], 2), "initial_misfit": h_plain[0][1],
-                  "final_misfit": h_plain[-1][1], "iterations": len(h_plain) - 1},
-        "learned": {"seconds_to_target": round(t_learn, 2) if t_learn is not None else None,
-                    "final_misfit": h_learn[-1][1], "iterations": len(h_learn) - 1,
-                    "fallbacks": int(sum(1 for h in h_learn if h[2]))},
+                   "widths": list(widths), "parameters": params, "seeds": a.c5_seeds,
+                   "holdout_models": a.c5_holdout, "epochs": a.c5_epochs,
+                   "fwi_iterations": a.c5_iterations},
+        "ratio_min": min(ratios), "ratio_max": max(ratios) if np.isfinite(max(ratios)) else None,
+        "runs_not_reaching_target": int(sum(1 for r in ratios if not np.isfinite(r))),
+        "runs_at_or_below_half": int(sum(1 for r in ratios if r <= 0.5)),
+        "pairs_seconds": round(t_pairs, 1), "train_seconds": [round(t, 1) for t in t_train],
+        "train_loss_final": [round(h[-1], 5) for h in hist],
+        "runs": runs,
     }
 
 
-def _unet_flops(h, w, batch, base=32):
-    """Multiply-add FLOPs of one forward pass of the U-Net on batch images
-    of h x w (padded) pixels: 2 * pixels * K * N per convolution."""
-    c = [base, 2 * base, 4 * base, 8 * base]
+def _unet_flops(h, w, batch, widths):
+    """Multiply-add FLOPs of one forward pass of the U-Net with channel
+    widths `widths` on batch images of h x w (padded) pixels:
+    2 * pixels * K * N per convolution."""
+    c = list(widths)
     f, hh, ww = 0, h, w
     ins = [3, c[0], c[1]]
     for lvl in range(3):
@@ -469,7 +548,9 @@ def unet_kernels(a):
            "hip_infer_ms": timed(lambda: unet(x1), n),
            "torch_infer_ms": timed(torch_infer, n)}
     _progress(" ".join(f"{k} {v:.3f}" for k, v in res.items()))
-    flops = 3 * _unet_flops(320, 192, batch)
+    widths = [net.enc[0][0].out_channels, net.enc[1][0].out_channels,
+              net.enc[2][0].out_channels, net.mid[0].out_channels]
+    flops = 3 * _unet_flops(320, 192, batch, widths)
     tf = flops / (res["hip_train_ms"] * 1e-3) / 1e12
     return {
         "metric": "U-Net training step time (bf16 MFMA kernels)",
@@ -477,7 +558,8 @@ def unet_kernels(a):
         "higher_is_better": False, "n_gpus": 1, "steps": n, "warmup": a.warmup,
         "dtype": "bf16 operands, fp32 accumulation", "data": "synthetic (random gradients)",
         "config": {"workload": "C5 preconditioner network", "batch": batch,
-                   "grid": f"{nx}x{nz} padded to 320x192", "parameters": 1.9e6},
+                   "grid": f"{nx}x{nz} padded to 320x192", "widths": widths,
+                   "parameters": sum(p.numel() for p in net.parameters())},
         "torch_autocast_train_ms": round(res["torch_train_ms"], 3),
         "speedup_vs_torch_train": round(res["torch_train_ms"] / res["hip_train_ms"], 3),
         "hip_infer_ms": round(res["hip_infer_ms"], 3),
@@ -554,15 +636,23 @@ def main():
     for k in range(a.warmup):
         step()
         _progress(f"warmup step {k + 1}/{a.warmup}")
-    model.set_timing(a.timing_stride)
+    # the per-shot calls run asynchronously only without step timing, so
+    # the call-by-call driver is timed by the wall clock alone
+    events = not a.sequential or a.graph or a.timing_stride > 0
+    model.set_timing(a.timing_stride if events else 0)
     model.reset_step_timing()
     t0 = time.perf_counter()
     for k in range(a.steps):
-        step()  # every call synchronises the model stream before returning
+        step()  # the step's last call waits for the device
         _progress(f"timed step {k + 1}/{a.steps}")
     secs = time.perf_counter() - t0
-    kern_ms, timed, launches = model.step_timing()
+    if events:
+        kern_ms, timed, launches = model.step_timing()
+    else:
+        launches = timed = 2 * model.n_shots * model.nt * a.steps
+        kern_ms = secs * 1e3
     model.set_timing(0)
+    alone_us = kernel_alone_us(model)
 
     nx, nz, nt = model.nx, model.nz, model.nt
     points_per_step = 2.0 * model.n_shots * nx * nz * nt
@@ -574,7 +664,8 @@ def main():
         nx, nz, nt, cfg["inversion"]["snapshot_interval"], kpts) / (2.0 * nt)
     launches_per_kernel = {2: 2, 5: 2, 6: 0.5, 7: 0.5, 14: 0.5}.get(a.variant, 1)
     avg_launch_s = kern_ms / 1e3 / max(timed, 1)
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    device_achieved = bytes_per_launch / avg_launch_s / 1e9
+    achieved = bytes_per_launch / (alone_us * 1e-6) / 1e9
     kernel_name = {0: "step_fused<8,ieee>", 1: "step_fused<16,ieee>",
                    2: "step_split_stress<8,ieee>+step_split_velocity<8,ieee>",
                    3: "step_fused<8,recip>", 4: "step_fused<16,recip>",
@@ -582,7 +673,10 @@ def main():
                    6: "step_two<8,256>", 7: "step_two<16,512>",
                    8: "step_lean<16>", 9: "step_lean<8>", 10: "step_lean<12>",
                    11: "step_lean<20>", 12: "step_lean<24>", 13: "step_lean<32>",
-                   14: "step_lean2<8>",
+                   14: "step_lean2<8>", 15: "step_lean_pipe<8,2>",
+                   16: "step_lean_pipe<8,3>", 17: "step_lean_pipe<8,4>",
+                   18: "step_lean<4>", 19: "step_lean<8,6 waves>",
+                   20: "step_lean<12,5 waves>",
                    }[a.variant]
     out = {
         "metric": METRIC,
@@ -608,8 +702,11 @@ def main():
             "step_kernel": kernel_name,
             "xcd_remap": not a.no_xcd_remap,
             "hip_graph": a.graph,
-            "driver": ("call by call, one stream" if a.sequential or a.graph or
-                       a.timing_stride > 0 else
+            "driver": ("call by call, one stream" if a.graph or a.timing_stride > 0 else
+                       "call by call (forward_simulate per shot, misfit, adjoint "
+                       "sources, adjoint_simulate per shot): asynchronous calls, "
+                       "forwards two at a time, adjoints in pairs with deferred "
+                       "kernel terms" if a.sequential else
                        "run_model on two HIP streams: forwards of shots 1.. "
                        "beside the adjoints of shots 0.., the last adjoint's "
                        "kernel terms deferred and replayed in order (bitwise "
@@ -623,18 +720,36 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": _measured_traffic(kernel_name),
+            "traffic_note": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, each "
+                             "scaled by the same counter on a plain copy kernel: "
+                             "bytes between L2 and the memory fabric, Infinity-"
+                             "Cache hits included"),
             "algorithmic_bytes_per_launch": bytes_per_launch,
-            "avg_launch_us": round(avg_launch_s * 1e6, 3),
-            "timed_launches": timed,
-            "launches": launches,
-            "launches_per_time_step": launches_per_kernel,
-            "timing": (f"kernel start/end of every {a.timing_stride}th launch"
-                       if a.timing_stride > 0 else
-                       "HIP events around each time loop, per time step, gaps "
-                       "included" if a.sequential or a.graph else
-                       "HIP events around each run_model (both streams), per "
-                       "time step of every shot: wall time per launch while "
-                       "two launches overlap"),
+            "kernel_us": round(alone_us, 3),
+            "timing": ("kernel alone: HIP start/end events of every step launch "
+                       "(hipExtLaunchKernel) of one shot's forward + adjoint on "
+                       "the model's stream, after the timed region"),
+            "device_aggregate": {
+                "achieved": round(device_achieved, 1),
+                "frac": round(device_achieved / HBM_PEAK_GBS, 4),
+                "wall_us_per_launch": round(avg_launch_s * 1e6, 3),
+                "timed_launches": timed,
+                "launches": launches,
+                "launches_per_time_step": launches_per_kernel,
+                "timing": (f"kernel start/end of every {a.timing_stride}th launch"
+                           if a.timing_stride > 0 else
+                           "HIP events around each time loop, per time step, gaps "
+                           "included" if a.graph else
+                           "wall clock of the timed steps per time step launched"
+                           if a.sequential else
+                           "HIP events around each run_model (both streams), per "
+                           "time step of every shot: wall time per launch while "
+                           "two launches overlap"),
+                "note": ("one stream" if a.graph or a.timing_stride > 0 else
+                         "two time loops at a time share the GPU; their two "
+                         "state sets and materials (2 x 84 MB) fit the 256 MB "
+                         "Infinity Cache, so part of this rate is served by it"),
+            },
         },
     }
     if copy:

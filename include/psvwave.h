@@ -16,8 +16,13 @@
  *  - Host buffers are caller-owned and use the reference layout: grids
  *    (nx, nz) with iz fastest, data (n_shots, nr, nt), snapshots
  *    (n_shots, snapshots, nx, nz), vectors (free_parameters,).
- *  - Calls are synchronous: results are complete when a call returns.  A
- *    model is not re-entrant; distinct models are independent.
+ *  - psv_forward and psv_adjoint enqueue their time loops on the device and
+ *    return (host-side argument errors are still reported at once); every
+ *    other call on the model first waits for them, so results are complete
+ *    when a call that returns them returns, and an error of an enqueued loop
+ *    is reported by the next call.  psv_set_async(m, 0) makes them
+ *    synchronous.  A model is not re-entrant; distinct models are
+ *    independent.
  */
 #ifndef PSVWAVE_H
 #define PSVWAVE_H
@@ -118,6 +123,18 @@ int psv_forward(psv_model *m, int i_shot, int store_fields, int verbose,
                 int output_wavefields);
 /* adjoint_simulate -- src/fdModel.cpp:789-969, psvWave.cpp:160-205. */
 int psv_adjoint(psv_model *m, int i_shot, int verbose);
+/* Asynchronous per-shot calls (no reference counterpart: the reference's
+ * calls return when their loop has finished).  With on = 1 (the default)
+ * psv_forward and psv_adjoint return once their time loop is enqueued:
+ * consecutive forwards run two at a time on two HIP streams, and
+ * consecutive adjoints pair up, the second of a pair running beside the
+ * first with its kernel terms deferred to a buffer (3 nx nz doubles per
+ * snapshot) and added afterwards in its own order -- outputs are bitwise
+ * those of synchronous calls.  Verbose calls, step timing, graph mode,
+ * checkpointing and step variants 2, 5, 6, 7 and 14 stay synchronous.
+ * psv_synchronize waits for the enqueued loops. */
+int psv_set_async(psv_model *m, int on);
+int psv_synchronize(psv_model *m);
 /* calculate_l2_misfit -- src/fdModel.cpp:1128-1145.  Also stored as the
  * misfit attribute. */
 int psv_calculate_l2_misfit(psv_model *m, double *misfit);
@@ -186,7 +203,11 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * step kernel with 64x8 tiles, 1 fused 64x16, 2 split (two launches per
  * step), 3-5 the same with reciprocal-corrected divisions, 6 / 7 two steps
  * per launch (64x8 / 64x16 tiles), 8-13 lean one-step kernels (64 x 16,
- * 8, 12, 20, 24, 32 tiles; the default when dx, dz lie in [2^-40, 2^40]).
+ * 8, 12, 20, 24, 32 tiles; 9 is the default when dx, dz lie in [2^-40,
+ * 2^40]), 14 the lean two-step kernel, 15-17 lean kernels whose blocks
+ * advance 2, 3 or 4 tiles of a column strip in turn (the next tile's loads
+ * in flight meanwhile), 18 the lean kernel on 64 x 4 tiles, 19 / 20 the
+ * 64 x 8 / 64 x 12 lean kernels held to 6 / 5 waves per SIMD.
  * All variants give bitwise identical results.  xcd_remap: 1 keeps
  * neighbouring tiles on one XCD.
  * timing_stride k > 0 records the kernel start and end of every k-th step
@@ -261,8 +282,19 @@ int psv_batch_misfits(psv_batch *b, double *out);
 /* Step kernel for all members: 0, 1, 6, 7 or 8-13 (see
  * psv_set_step_variant). */
 int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
+/* FWI gradient of every member against its observed data (what
+ * psv_run_model(m, 0, 1) does for one model): kernels reset, then shot by
+ * shot the forward storing snapshots, the adjoint sources and the adjoint
+ * with kernel correlation -- one launch per time step over every member --
+ * then misfits (psv_batch_misfits) and velocity kernels
+ * (psv_batch_get_kernels, (nx, nz) each).  Needs one snapshot_interval for
+ * all members, dividing nt, and a one-step variant; each member keeps one
+ * shot's snapshots.  Results equal each member's run_model bit for bit. */
+int psv_batch_gradients(psv_batch *b);
+int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
+                          double *rho_k);
 int psv_batch_set_graph_mode(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13 (ms_per_step[10],
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20 (ms_per_step[13],
  * -1 for a variant the batch cannot use) and keeps the fastest (*best).
  * Overwrites shot 0's synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,

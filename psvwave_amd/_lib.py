@@ -108,6 +108,8 @@ _SIGNATURES = {
     "psv_set_timing": [_P, _i],
     "psv_set_graph_mode": [_P, _i],
     "psv_set_checkpointing": [_P, _i],
+    "psv_set_async": [_P, _i],
+    "psv_synchronize": [_P],
     "psv_get_step_timing": [_P, _D, C.POINTER(C.c_int64),
                             C.POINTER(C.c_int64)],
     "psv_reset_step_timing": [_P],
@@ -124,6 +126,8 @@ _SIGNATURES = {
     "psv_batch_get_synthetic_data": [_P, _i, _D, _D],
     "psv_batch_forward": [_P, _i],
     "psv_batch_misfits": [_P, _D],
+    "psv_batch_gradients": [_P],
+    "psv_batch_get_kernels": [_P, _i, _D, _D, _D],
     "psv_batch_set_variant": [_P, _i, _i],
     "psv_batch_set_graph_mode": [_P, _i],
     "psv_batch_autotune": [_P, _i, C.POINTER(C.c_int), _D],

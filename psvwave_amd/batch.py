@@ -14,7 +14,7 @@ from . import _lib
 from .model import _f64
 
 # step variants Batch.autotune times, in the library's order
-AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13)
+AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20)
 
 
 class Batch:
@@ -85,6 +85,24 @@ class Batch:
         _lib.check(self._lib.psv_batch_misfits(self._h, _lib.dptr(out)))
         return out
 
+    def gradients(self):
+        """FWI gradient of every member against its observed data (each
+        member's run_model: forward with snapshots, misfit, adjoint sources,
+        kernel reset, adjoint with correlation, kernel mapping), shot by
+        shot with one launch per time step over all members.  Returns the
+        misfits; the velocity kernels come from get_kernels(m)."""
+        _lib.check(self._lib.psv_batch_gradients(self._h))
+        out = np.empty(len(self))
+        _lib.check(self._lib.psv_batch_misfits(self._h, _lib.dptr(out)))
+        return out
+
+    def get_kernels(self, m):
+        """Velocity kernels (vp, vs, rho) of member m after gradients()."""
+        i = self.infos[m]
+        a = [np.empty((i["nx"], i["nz"])) for _ in range(3)]
+        _lib.check(self._lib.psv_batch_get_kernels(self._h, int(m), *map(_lib.dptr, a)))
+        return tuple(a)
+
     def set_variant(self, variant, xcd_remap=True):
         _lib.check(self._lib.psv_batch_set_variant(self._h, int(variant),
                                                    int(bool(xcd_remap))))
@@ -93,7 +111,7 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 
     def autotune(self, steps=20):
-        """Time variants 0, 1, 6, 7, 8-13 over `steps` steps and keep the fastest.
+        """Time variants 0, 1, 6, 7, 8-13, 18-20 over `steps` steps and keep the fastest.
         Returns (best variant, {variant: ms per step})."""
         best = C.c_int()
         ms = np.zeros(len(AUTOTUNE_VARIANTS))

@@ -10,8 +10,9 @@
 
 namespace psv {
 
-// Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9, 10, 11, 12, 13.
-constexpr int kBatchCandidates = 10;
+// Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9, 10, 11, 12, 13,
+// 18, 19, 20.
+constexpr int kBatchCandidates = 13;
 
 class Batch {
  public:
@@ -35,6 +36,14 @@ class Batch {
   void forward(int i_shot);
   // Per-member L2 misfit over all shots (calculate_l2_misfit of each).
   void misfits(double *out);
+  // FWI gradient of every member against its observed data (the reference's
+  // run_model, src/fdModel.cpp:1291-1308): kernels reset, then shot by shot
+  // the forward storing snapshots, the adjoint sources and the adjoint with
+  // kernel correlation, each one launch per time step over every member's
+  // tiles; then misfits and the kernel mapping.  Each member keeps one
+  // shot's snapshots (5 nx nz doubles per snapshot).  Kernels and misfits
+  // equal the members' own run_model bit for bit.
+  void gradients();
   // Times `steps` batched steps of each candidate variant (ms_per_variant
   // gets kBatchCandidates entries, -1 for a variant the batch cannot use),
   // keeps the fastest; returns its time per step in ms.
@@ -47,6 +56,8 @@ class Batch {
   void build_tables();
   void enqueue_loop(int i_shot, int nsteps);
   void clear_graphs();
+  void clear_gradient_tables();
+  void enqueue_gradient_loop(int i_shot, bool forward);
 
   std::vector<std::unique_ptr<Model>> models_;
   int nt_ = 0, n_shots_ = 0, variant_ = kFusedTz8, xcd_remap_ = 1;
@@ -59,6 +70,9 @@ class Batch {
   std::vector<hipGraphExec_t> graphs_;  // per shot
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double loop_ms_ = 0.0;
+  // gradient descriptors [model][shot][parity] with snapshot-store bases
+  StepArgs *gfwd_ = nullptr, *gadj_ = nullptr;
+  int si_ = 1;
 };
 
 }  // namespace psv

@@ -45,6 +45,7 @@ Batch::~Batch() {
   clear_graphs();
   if (descs_) (void)hipFree(descs_);
   if (tiles_) (void)hipFree(tiles_);
+  clear_gradient_tables();
   for (auto e : ev_)
     if (e) (void)hipEventDestroy(e);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -71,17 +72,24 @@ void Batch::set_graph_mode(bool on) {
   graph_mode_ = on;
 }
 
+void Batch::clear_gradient_tables() {
+  if (gfwd_) (void)hipFree(gfwd_);
+  if (gadj_) (void)hipFree(gadj_);
+  gfwd_ = gadj_ = nullptr;
+}
+
 void Batch::set_variant(int variant, int xcd_remap) {
   if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
       variant != kTwoStepTz16 && !step_is_lean(variant))
-    throw ValueError("batched runs use step variants 0, 1, 6, 7 or 8-13");
+    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8-13 or 18-20");
   if (step_is_lean(variant))
     for (auto &m : models_)
       if (!m->lean_ok())
-        throw ValueError("a member of the batch cannot use step variants 8-13");
+        throw ValueError("a member of the batch cannot use the lean step variants");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
   clear_graphs();
+  clear_gradient_tables();  // descriptors hold injection-table pointers
   build_tables();
 }
 
@@ -125,7 +133,7 @@ void Batch::enqueue_loop(int i_shot, int nsteps) {
   for (int k = 0; k < nsteps;) {
     const bool pair = two && k + 1 < nsteps;
     BatchLaunch b{descs_, tiles_, ntiles_, n_shots_, i_shot, launch & 1,
-                  k, k + 1, xcd_remap_};
+                  k, k + 1, xcd_remap_, 0, 1};
     hip_check(launch_batch_step(variant_, pair, b, stream_), "batch step");
     k += pair ? 2 : 1;
     ++launch;
@@ -172,6 +180,71 @@ void Batch::forward(int i_shot) {
   loop_ms_ += ms;
 }
 
+// Time loop of shot i_shot for every member with the gradient descriptors:
+// forward storing each member's snapshots of this shot, or adjoint (time
+// steps nt-1 .. 0) correlating with them.
+void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
+  for (auto &m : models_) m->enqueue_zero_state(stream_);
+  const StepArgs *descs = forward ? gfwd_ : gadj_;
+  for (int k = 0; k < nt_; ++k) {
+    const int it = forward ? k : nt_ - 1 - k;
+    BatchLaunch b{descs, tiles_, ntiles_, n_shots_, i_shot, k & 1, it, it, xcd_remap_,
+                  1, si_};
+    hip_check(launch_batch_step(variant_, false, b, stream_, nullptr, nullptr,
+                                !forward && it % si_ == 0),
+              "batch gradient step");
+  }
+}
+
+void Batch::gradients() {
+  if (step_is_two_step(variant_))
+    throw ValueError("batched gradients need a one-step variant (0, 1, 8-13, 18-20)");
+  si_ = models_[0]->config().snapshot_interval;
+  for (auto &m : models_)
+    if (m->config().snapshot_interval != si_)
+      throw ValueError("batched gradients need one snapshot_interval for all members");
+  if (nt_ % si_ != 0)
+    throw ValueError("Snapshot interval and size of accumulator don't match! "
+                     "(nt must be a multiple of snapshot_interval)");
+  if (!gfwd_) {
+    std::vector<StepArgs> f, a;
+    for (auto &m : models_) {
+      double *const *store = m->shot_snapshot_store();
+      for (int s = 0; s < n_shots_; ++s)
+        for (int p = 0; p < 2; ++p) {
+          f.push_back(m->forward_launch_args(s, p));
+          a.push_back(m->adjoint_launch_args(s, p));
+          for (int q = 0; q < 5; ++q) f.back().snap[q] = a.back().snap[q] = store[q];
+        }
+    }
+    hip_check(hipMalloc((void **)&gfwd_, f.size() * sizeof(StepArgs)), "hipMalloc");
+    hip_check(hipMalloc((void **)&gadj_, a.size() * sizeof(StepArgs)), "hipMalloc");
+    hip_check(hipMemcpy(gfwd_, f.data(), f.size() * sizeof(StepArgs), hipMemcpyHostToDevice),
+              "upload");
+    hip_check(hipMemcpy(gadj_, a.data(), a.size() * sizeof(StepArgs), hipMemcpyHostToDevice),
+              "upload");
+  }
+  for (auto &m : models_) m->reset_kernels();
+  hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+  for (int s = 0; s < n_shots_; ++s) {
+    // forward s, its adjoint sources (each member's own stream), adjoint s:
+    // the kernels accumulate shot by shot, as in the reference's order
+    enqueue_gradient_loop(s, true);
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (auto &m : models_) m->calculate_l2_adjoint_sources();
+    enqueue_gradient_loop(s, false);
+  }
+  hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+  loop_ms_ += ms;
+  for (auto &m : models_) {
+    m->calculate_l2_misfit();
+    m->map_kernels_to_velocity();
+  }
+}
+
 void Batch::misfits(double *out) {
   for (size_t m = 0; m < models_.size(); ++m)
     out[m] = models_[m]->calculate_l2_misfit();
@@ -181,7 +254,8 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
   if (steps < 2) throw ValueError("autotune needs at least 2 steps");
   const int candidates[kBatchCandidates] = {
       kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16, kLeanTz16,
-      kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32};
+      kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32,
+      kLeanTz4,  kLeanTz8W6, kLeanTz12W5};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;

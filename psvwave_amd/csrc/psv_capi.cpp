@@ -38,11 +38,20 @@ int guard(F &&f) {
   }
 }
 
+// The model, settled: every entry point but forward and adjoint first waits
+// for the time loops those enqueue (and surfaces their errors).
 psv::Model &M(psv_model *m) {
   if (!m || !m->impl) throw psv::ValueError("null model handle");
+  m->impl->settle();
   return *m->impl;
 }
 const psv::Model &M(const psv_model *m) {
+  if (!m || !m->impl) throw psv::ValueError("null model handle");
+  m->impl->settle();
+  return *m->impl;
+}
+// The model as it is (forward and adjoint enqueue behind pending work).
+psv::Model &MA(psv_model *m) {
   if (!m || !m->impl) throw psv::ValueError("null model handle");
   return *m->impl;
 }
@@ -327,13 +336,13 @@ int psv_get_folders(const psv_model *m, char *observed, char *stf, int len) {
 int psv_forward(psv_model *m, int i_shot, int store_fields, int verbose,
                 int output_wavefields) {
   return guard([&] {
-    M(m).forward(i_shot, store_fields != 0, verbose != 0,
-                 output_wavefields != 0);
+    MA(m).forward(i_shot, store_fields != 0, verbose != 0,
+                  output_wavefields != 0);
   });
 }
 
 int psv_adjoint(psv_model *m, int i_shot, int verbose) {
-  return guard([&] { M(m).adjoint(i_shot, verbose != 0); });
+  return guard([&] { MA(m).adjoint(i_shot, verbose != 0); });
 }
 
 int psv_calculate_l2_misfit(psv_model *m, double *misfit) {
@@ -551,6 +560,14 @@ int psv_set_graph_mode(psv_model *m, int on) {
   return guard([&] { M(m).set_graph_mode(on != 0); });
 }
 
+int psv_set_async(psv_model *m, int on) {
+  return guard([&] { M(m).set_async(on != 0); });
+}
+
+int psv_synchronize(psv_model *m) {
+  return guard([&] { M(m); });
+}
+
 int psv_set_checkpointing(psv_model *m, int every) {
   return guard([&] { M(m).set_checkpointing(every); });
 }
@@ -717,6 +734,20 @@ int psv_batch_misfits(psv_batch *b, double *out) {
   return guard([&] {
     need(out, "out");
     B(b).misfits(out);
+  });
+}
+
+int psv_batch_gradients(psv_batch *b) {
+  return guard([&] { B(b).gradients(); });
+}
+
+int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
+                          double *rho_k) {
+  return guard([&] {
+    need(vp_k, "vp_k");
+    need(vs_k, "vs_k");
+    need(rho_k, "rho_k");
+    B(b).model(m).get_kernels(vp_k, vs_k, rho_k);
   });
 }
 

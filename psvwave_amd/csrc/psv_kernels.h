@@ -179,7 +179,10 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 // 3-5 are the same kernels with RecipDiv (bit-identical results), 6-7
 // advance two time steps per launch, 8-13 are the lean one-step kernels
 // (64 x 16, 8, 12, 20, 24, 32 tiles, FastDiv with IEEE fallback), 14 the
-// lean two-step kernel (64 x 8 tiles).
+// lean two-step kernel (64 x 8 tiles), 15-17 the pipelined lean kernels,
+// 18 the lean kernel on 64 x 4 tiles (7 waves per SIMD), 19 and 20 the
+// 64 x 8 and 64 x 12 lean kernels held to 6 and 5 waves per SIMD, 21 the
+// 64 x 8 lean kernel with the velocity halo columns in the row transfers.
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -196,15 +199,25 @@ enum StepVariant {
   kLeanTz24 = 12,
   kLeanTz32 = 13,
   kLeanTwoTz8 = 14,
-  kNumVariants = 15
+  kLeanPipe2Tz8 = 15,
+  kLeanPipe3Tz8 = 16,
+  kLeanPipe4Tz8 = 17,
+  kLeanTz4 = 18,
+  kLeanTz8W6 = 19,
+  kLeanTz12W5 = 20,
+  kLeanTz8Halo = 21,
+  kNumVariants = 22
 };
 constexpr int kMaxTileHeight = 32;
 
 // Lean one-step kernels (variants 8-13): lane = column, FastDiv, reduced
 // stress ring.  They need FastDiv's divisor range and a taper table that is
 // nondecreasing in the nesting count; the host checks both.
-bool step_is_lean(int variant);       // lean one-step kernels (8-13)
+bool step_is_lean(int variant);       // lean one-step kernels (8-13, 18-21)
 bool step_uses_fastdiv(int variant);  // lean one- and two-step kernels
+// Pipelined lean kernels (15-17): 64 x 8 tiles, a block advances 2, 3 or 4
+// tiles of one column strip in turn, fetching the next while one computes.
+bool step_is_lean_pipe(int variant);
 bool fast_division_ok(double d);
 void fast_division_constants(double d, double *zh, double *zl);
 
@@ -222,14 +235,18 @@ hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t stream,
 // One time step (or two, for the two-step variants) of every tile of a batch
 // of models: descs holds per (model, shot, parity) launch arguments, tiles
 // the (model, tile) pairs of all models.
+// With `snapshots`, step it of a member stores (forward) or correlates with
+// (adjoint; launched with corr) snapshot slot it / si when si divides it:
+// the descriptor's snap pointers are the bases of a one-shot snapshot store.
 struct BatchLaunch {
   const StepArgs *descs;
   const int2 *tiles;
   int ntiles, n_shots, shot, parity, it, it1, xcd_remap;
+  int snapshots, si;
 };
 hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
                              hipStream_t stream, hipEvent_t e0 = nullptr,
-                             hipEvent_t e1 = nullptr);
+                             hipEvent_t e1 = nullptr, bool corr = false);
 // bfloat16 MFMA kernels of the preconditioner network (psv_conv.hip).
 // NHWC bfloat16 activations with images stacked along the rows.
 enum ConvMode { kConv = 0, kScatter = 1, kGather2x2 = 2 };

@@ -468,6 +468,21 @@ __device__ __forceinline__ void dma_row(const double *base, unsigned lane_off,
                                    4, 0, 0);
 }
 
+// One staged velocity row, columns -3 .. 66 (70 doubles = 140 dwords),
+// from memory straight into LDS: dwords 0-63, 64-127 and 128-139 (12
+// lanes).  row_base is the global address of column -3, lds_row the LDS
+// address of column -3 (both wave-uniform).
+__device__ __forceinline__ void dma_row70(const double *row_base, double *lds_row) {
+  const unsigned lane = threadIdx.x & 63;
+  const char *g = (const char *)row_base + lane * 4u;
+  __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)lds_row, 4, 0, 0);
+  __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(lds_row + 32), 4,
+                                   0, 0);
+  if (lane < 12)
+    __builtin_amdgcn_global_load_lds((const void *)(g + 512), (lds_void *)(lds_row + 64), 4,
+                                     0, 0);
+}
+
 __device__ __forceinline__ double ldb(const double *base, unsigned off) {
   return *(const double *)((const char *)base + off);
 }
@@ -512,7 +527,9 @@ __device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
   return (unsigned)(r + 3) * rowb + (unsigned)(c + 8) * 8u;
 }
 
-template <int TZ, bool CORR>
+// HALO_DMA: the halo columns of the velocity rows come in the rows' own
+// transfers (70-column rows) instead of through registers.
+template <int TZ, bool CORR, bool HALO_DMA = false>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, const int it) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
@@ -545,9 +562,15 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   for (int k = 0; k < L::KV; ++k) {
     const int r = w + 4 * (k - 1);
     if (r >= -3 && r <= TZ + 2) {
-      const unsigned o = off(r, lane);
-      dma_row(vxi, o, s_vx + L::V(r, 0));
-      dma_row(vzi, o, s_vz + L::V(r, 0));
+      if (HALO_DMA) {
+        const long long ro = tb + (long long)(r + 3) * a.pitch + 5;  // column -3
+        dma_row70(a.vx_i + ro, s_vx + L::V(r, -3));
+        dma_row70(a.vz_i + ro, s_vz + L::V(r, -3));
+      } else {
+        const unsigned o = off(r, lane);
+        dma_row(vxi, o, s_vx + L::V(r, 0));
+        dma_row(vzi, o, s_vz + L::V(r, 0));
+      }
     }
   }
   double vxs[2], vzs[2];
@@ -557,7 +580,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     const int t = threadIdx.x + 256 * k;
     sidx[k] = -1;
     vxs[k] = vzs[k] = 0.0;
-    if (k * 256 < L::NSIDE && t < L::NSIDE) {
+    if (!HALO_DMA && k * 256 < L::NSIDE && t < L::NSIDE) {
       int r = t / 6 - 3, cs = t - (t / 6) * 6;
       int c = cs < 3 ? cs - 3 : 61 + cs;
       unsigned o = off(r, c);
@@ -854,12 +877,413 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
 
 // CORR: instantiation for adjoint snapshot steps (kernel correlation; one
 // wave per SIMD fewer, for the correlation operands in flight).
-template <int TZ, int WPE, bool CORR>
+template <int TZ, int WPE, bool CORR, bool HALO_DMA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ, CORR>(a, tile, a.it);
+  lean_tile<TZ, CORR, HALO_DMA>(a, tile, a.it);
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined lean kernel: a block advances P tiles of 64 x TZ that are
+// consecutive in z within one 64-column strip, one after the other, with
+// the loads of tile k + 1 in flight while tile k computes.  Vector memory
+// counters retire in issue order, so tile k's stress and velocity passes
+// contain no global loads (but the rare IEEE redo): the velocity rows of a
+// tile, halo columns included, go by DMA into one of two LDS buffers --
+// tile k + 1's as soon as tile k's have landed -- and every other operand
+// (own stresses and materials, ring row and ring-column items, taper
+// factors, correlation operands) is fetched into registers in one batch,
+// tile k + 1's once tile k's stresses sit in LDS, so the batch overlaps
+// tile k's velocity pass.  A tile's injection entries are staged at its
+// first barrier, before the next tile's transfers are queued; its velocity
+// stores are issued after the next tile's first barrier.  Per point the
+// arithmetic, and so every result, is that of lean_tile.
+template <int TZ, bool CORR>
+struct LeanRegs {
+  static constexpr int R = TZ / 4, NCP = Lean<TZ>::NCP;
+  double oxx[R], ozz[R], oxz[R], olm[R], omu[R], ob[R];
+  double trow[R], tx;
+  double gzz, gxz, glm, gmu, gt;
+  double cxx[NCP], cxz[NCP], clm[NCP], cmu[NCP], ct[NCP];
+  bool cin[R];
+  double csv[CORR ? R : 1][5], crk[CORR ? R : 1], clk[CORR ? R : 1], cmk[CORR ? R : 1];
+};
+
+// Velocity rows z0-3 .. z0+TZ+2, columns x0-3 .. x0+66 of `tile` into the
+// LDS buffer `buf` (vx, then vz) -- the halo columns come in the same
+// transfers as the row.
+template <int TZ>
+__device__ __forceinline__ void lean_dma_tile(const StepArgs &a, int tile, double *buf) {
+  using L = Lean<TZ>;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 3);
+  double *const s_vx = buf, *const s_vz = buf + L::VR * L::VW;
+#pragma unroll
+  for (int k = 0; k < L::KV; ++k) {
+    const int r = w + 4 * (k - 1);
+    if (r >= -3 && r <= TZ + 2) {
+      const long long ro = tb + (long long)(r + 3) * a.pitch;
+      dma_row70(a.vx_i + ro, s_vx + L::V(r, -3));
+      dma_row70(a.vz_i + ro, s_vz + L::V(r, -3));
+    }
+  }
+}
+
+// Every register operand of `tile` at step `it` (see LeanRegs).
+template <int TZ, bool CORR>
+__device__ __forceinline__ void lean_fetch_tile(const StepArgs &a, int tile, int it,
+                                                LeanRegs<TZ, CORR> &g) {
+  using L = Lean<TZ>;
+  constexpr int R = L::R;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
+  const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 8);
+  const unsigned rowb = (unsigned)a.pitch * 8u;
+  auto off = [&](int r, int c) -> unsigned { return lean_off(r, c, rowb); };
+  const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
+  const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
+  auto taper_row = [&](int z) {
+    return a.taper_z[z < 0 ? 0 : (z >= a.nz ? a.nz - 1 : z)];
+  };
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    unsigned o = off(w + 4 * j, lane);
+    g.oxx[j] = ldb(txxi, o);
+    g.ozz[j] = ldb(tzzi, o);
+    g.oxz[j] = ldb(txzi, o);
+    g.olm[j] = ldb(lmp, o);
+    g.omu[j] = ldb(mup, o);
+    g.ob[j] = ldb(bp, o);
+    g.trow[j] = taper_row(z0 + w + 4 * j);
+  }
+  const int x = x0 + lane;
+  g.tx = a.taper_x[x < a.nx ? x : 0];
+  if (CORR) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int z = z0 + w + 4 * j;
+      g.cin[j] = x < a.nx && z < a.nz && x >= a.kx0 && x < a.kx1 && z >= a.kz0 && z < a.kz1;
+      if (g.cin[j]) {
+        const long long ci = (long long)z * a.nx + x;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) g.csv[j][f] = a.snap[f][ci];
+        g.crk[j] = a.rho_k[ci];
+        g.clk[j] = a.lam_k[ci];
+        g.cmk[j] = a.mu_k[ci];
+      }
+    }
+  }
+  const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
+  {
+    unsigned o = off(rr, lane);
+    g.gzz = g.gxz = g.glm = 0.0;
+    if (w != 3) g.gxz = ldb(txzi, o);
+    if (w != 0) {
+      g.gzz = ldb(tzzi, o);
+      g.glm = ldb(lmp, o);
+    }
+    g.gmu = ldb(mup, o);
+    g.gt = taper_row(z0 + rr);
+  }
+#pragma unroll
+  for (int p = 0; p < L::NCP; ++p) {
+    g.cxx[p] = g.cxz[p] = g.clm[p] = g.cmu[p] = g.ct[p] = 0.0;
+    const int i = p * 64 + lane;
+    if ((p & 3) == w && i < L::NCI) {
+      const int q = i / TZ, r = i - q * TZ;
+      const int c = q < 2 ? q - 2 : 62 + q;
+      unsigned o = off(r, c);
+      g.cxx[p] = ldb(txxi, o);
+      g.cxz[p] = ldb(txzi, o);
+      g.clm[p] = ldb(lmp, o);
+      g.cmu[p] = ldb(mup, o);
+      const int xc = x0 + c;
+      g.ct[p] = fmin(a.taper_x[xc < 0 ? 0 : (xc >= a.nx ? a.nx - 1 : xc)], taper_row(z0 + r));
+    }
+  }
+}
+
+template <int TZ, int P, bool CORR>
+__device__ __forceinline__ void lean_pipe_strip(const StepArgs &a, const int strip,
+                                                const int seg, const int it) {
+  using L = Lean<TZ>;
+  constexpr int R = L::R;
+  constexpr int NV = 2 * L::VR * L::VW;
+  constexpr int NS = TZ * L::XXW + L::ZZR * 64 + L::XZR * L::XZW;
+  constexpr int NB = NV > NS ? NV : NS;
+  __shared__ double lds[2 * NB];
+  __shared__ double s_ival[kInjStage], s_icoef[kInjStage];
+  __shared__ int s_iloc[kInjStage];
+
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int tz0 = seg * P;
+  const int nk = min(P, a.ntz - tz0);
+  const unsigned rowb = (unsigned)a.pitch * 8u;
+  auto off = [&](int r, int c) -> unsigned { return lean_off(r, c, rowb); };
+  const int x0 = strip * kTileX, x = x0 + lane;
+  const bool xin = x < a.nx;
+  const bool xint = x >= 2 && x < a.nx - 2;
+  const double dts = a.dts;
+  const IeeeDiv ix{a.dx}, iz{a.dz};
+  auto zint = [&](int z) { return z >= 2 && z < a.nz - 2; };
+
+  LeanRegs<TZ, CORR> g;
+  const int t_first = tz0 * a.ntx + strip;
+  lean_dma_tile<TZ>(a, t_first, lds);
+  lean_fetch_tile<TZ, CORR>(a, t_first, it, g);
+  if (!a.adjoint && t_first == 0) record_receivers(a, it);
+  // velocities of the previous tile, stored after the next tile's barrier
+  double pvx[R], pvz[R];
+  long long ptb = 0;
+  int pz0 = 0;
+  bool pend = false;
+
+  for (int k = 0; k < nk; ++k) {
+    const int tile = (tz0 + k) * a.ntx + strip;
+    const int z0 = (tz0 + k) * TZ;
+    double *const buf = lds + (k & 1) * NB;
+    double *const s_vx = buf, *const s_vz = buf + L::VR * L::VW;
+    double *const s_xx = buf, *const s_zz = buf + TZ * L::XXW;
+    double *const s_xz = s_zz + L::ZZR * 64;
+    const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 8);
+    __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS DMA and operands have landed
+    __syncthreads();
+    // this tile's injection entries and their samples into LDS, before the
+    // next tile's transfers are queued behind them (most tiles have none)
+    const int e0 = a.tile_off[tile], ne = a.tile_off[tile + 1] - e0;
+    if ((int)threadIdx.x < ne && (int)threadIdx.x < kInjStage) {
+      const InjEntry E = a.entries[e0 + threadIdx.x];
+      s_iloc[threadIdx.x] = E.local | (E.comp << 16);
+      s_icoef[threadIdx.x] = E.coef;
+      s_ival[threadIdx.x] = inj_sample(a, E, it);
+    }
+    if (k + 1 < nk) lean_dma_tile<TZ>(a, tile + a.ntx, lds + ((k + 1) & 1) * NB);
+    if (pend) {
+      double *vxo = a.vx_o + ptb, *vzo = a.vz_o + ptb;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        if (xin && pz0 + r < a.nz) {
+          const unsigned o = off(r, lane);
+          stb(vxo, o, pvx[j]);
+          stb(vzo, o, pvz[j]);
+        }
+      }
+    }
+    double vx0[R], vz0[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      vx0[j] = s_vx[L::V(w + 4 * j, lane)];
+      vz0[j] = s_vz[L::V(w + 4 * j, lane)];
+    }
+    const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
+    const double *lmp = a.lm + tb, *mup = a.mu + tb;
+    if (CORR) {
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (g.cin[j]) {
+          const long long ci = (long long)(z0 + w + 4 * j) * a.nx + x;
+          correlate_point(a.wcorr, g.olm[j], g.omu[j], vx0[j], vz0[j], g.oxx[j], g.ozz[j],
+                          g.oxz[j], g.csv[j][0], g.csv[j][1], g.csv[j][2], g.csv[j][3],
+                          g.csv[j][4], g.crk[j], g.clk[j], g.cmk[j]);
+          a.rho_k[ci] = g.crk[j];
+          a.lam_k[ci] = g.clk[j];
+          a.mu_k[ci] = g.cmk[j];
+        }
+    } else if (a.do_snap && !a.adjoint) {
+      lean_store_snapshot<R>(a, x, z0 + w, xin, vx0, vz0, g.oxx, g.ozz, g.oxz);
+    }
+    // ---- stresses (no global loads but the rare IEEE redo) ----
+    {
+      DivFlags fl;
+      const FastDiv fx{a.dx, a.rdx, a.zhx, a.zlx, &fl}, fz{a.dz, a.rdz, a.zhz, a.zlz, &fl};
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        if (xint && zint(z0 + r))
+          lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), g.olm[j], g.omu[j],
+                                              fmin(g.tx, g.trow[j]), dts, fx, fz, g.oxx[j],
+                                              g.ozz[j], g.oxz[j]);
+        row_fence();
+      }
+      if (!fl.ok()) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const int r = w + 4 * j;
+          const unsigned o = opaque(off(r, lane));
+          g.oxx[j] = ldb(txxi, o), g.ozz[j] = ldb(tzzi, o), g.oxz[j] = ldb(txzi, o);
+          if (xint && zint(z0 + r))
+            lean_stress<kXX | kZZ | kXZ, L::VW>(s_vx, s_vz, L::V(r, lane), ldb(lmp, o),
+                                                ldb(mup, o), fmin(g.tx, g.trow[j]), dts, ix,
+                                                iz, g.oxx[j], g.ozz[j], g.oxz[j]);
+        }
+      }
+    }
+    const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
+    {
+      const int z = z0 + rr;
+      if (xint && zint(z)) {
+        const double t = fmin(g.tx, g.gt);
+        const int v = L::V(rr, lane);
+        DivFlags gf;
+        const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
+        double hzz = g.gzz, hxz = g.gxz;
+        ring_row_stress<L::VW>(w, s_vx, s_vz, v, g.glm, g.gmu, t, dts, gx, gz, hzz, hxz);
+        if (!gf.ok()) {
+          const unsigned o = opaque(off(rr, lane));
+          hzz = ldb(tzzi, o), hxz = ldb(txzi, o);
+          ring_row_stress<L::VW>(w, s_vx, s_vz, v, ldb(lmp, o), ldb(mup, o), t, dts, ix, iz,
+                                 hzz, hxz);
+        }
+        g.gzz = hzz, g.gxz = hxz;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < L::NCP; ++p) {
+      const int i = p * 64 + lane;
+      if ((p & 3) == w && i < L::NCI) {
+        const int q = i / TZ, r = i - q * TZ;
+        const int c = q < 2 ? q - 2 : 62 + q;
+        const int xc = x0 + c, z = z0 + r;
+        if (xc >= 2 && xc < a.nx - 2 && zint(z)) {
+          DivFlags cf;
+          const FastDiv cx{a.dx, a.rdx, a.zhx, a.zlx, &cf}, cz{a.dz, a.rdz, a.zhz, a.zlz, &cf};
+          double hxx = g.cxx[p], hxz = g.cxz[p], hzz = 0.0;
+          lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), g.clm[p], g.cmu[p], g.ct[p],
+                                        dts, cx, cz, hxx, hzz, hxz);
+          if (!cf.ok()) {
+            const unsigned o = opaque(off(r, c));
+            hxx = ldb(txxi, o), hxz = ldb(txzi, o);
+            lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), ldb(lmp, o), ldb(mup, o),
+                                          g.ct[p], dts, ix, iz, hxx, hzz, hxz);
+          }
+          g.cxx[p] = hxx, g.cxz[p] = hxz;
+        }
+      }
+    }
+    double *txxo = a.txx_o + tb, *tzzo = a.tzz_o + tb, *txzo = a.txz_o + tb;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int r = w + 4 * j, z = z0 + r;
+      if (xin && z < a.nz) {
+        const unsigned o = off(r, lane);
+        stb(txxo, o, g.oxx[j]);
+        stb(tzzo, o, g.ozz[j]);
+        stb(txzo, o, g.oxz[j]);
+      }
+    }
+    __syncthreads();  // every wave is done reading this tile's velocity stage
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int r = w + 4 * j;
+      s_xx[L::XX(r, lane)] = g.oxx[j];
+      s_zz[L::ZZ(r, lane)] = g.ozz[j];
+      s_xz[L::XZ(r, lane)] = g.oxz[j];
+    }
+    if (w != 3) s_xz[L::XZ(rr, lane)] = g.gxz;
+    if (w != 0) s_zz[L::ZZ(rr, lane)] = g.gzz;
+#pragma unroll
+    for (int p = 0; p < L::NCP; ++p) {
+      const int i = p * 64 + lane;
+      if ((p & 3) == w && i < L::NCI) {
+        const int q = i / TZ, r = i - q * TZ;
+        const int c = q < 2 ? q - 2 : 62 + q;
+        if (q != 3) s_xx[L::XX(r, c)] = g.cxx[p];
+        if (q != 0) s_xz[L::XZ(r, c)] = g.cxz[p];
+      }
+    }
+    __syncthreads();
+    // the velocity pass needs only b, the tapers and the injection entries
+    // of this tile; the next tile's operands are fetched now
+    double ob[R], trow[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) ob[j] = g.ob[j], trow[j] = g.trow[j];
+    const double tx = g.tx;
+    if (k + 1 < nk) lean_fetch_tile<TZ, CORR>(a, tile + a.ntx, it, g);
+    {
+      DivFlags vf;
+      const FastDiv ux{a.dx, a.rdx, a.zhx, a.zlx, &vf}, uz{a.dz, a.rdz, a.zhz, a.zlz, &vf};
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        pvx[j] = vx0[j];
+        pvz[j] = vz0[j];
+        if (xint && zint(z0 + r))
+          lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ux, uz,
+                            pvx[j], pvz[j]);
+        row_fence();
+      }
+      if (!vf.ok()) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          const int r = w + 4 * j;
+          pvx[j] = vx0[j];
+          pvz[j] = vz0[j];
+          if (xint && zint(z0 + r))
+            lean_velocity<TZ>(s_xx, s_zz, s_xz, r, lane, ob[j], fmin(tx, trow[j]), dts, ix,
+                              iz, pvx[j], pvz[j]);
+        }
+      }
+    }
+    for (int e = 0; e < ne; ++e) {
+      int lc;
+      double coef, val;
+      if (e < kInjStage) {
+        lc = s_iloc[e], coef = s_icoef[e], val = s_ival[e];
+      } else {
+        const InjEntry E = a.entries[e0 + e];
+        lc = E.local | (E.comp << 16), coef = E.coef, val = inj_sample(a, E, it);
+      }
+      const int local = lc & 0xffff, comp = lc >> 16;
+      const int r = local / kTileX, c = local - r * kTileX;
+      if ((r & 3) != w || c != lane) continue;
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (j == (r >> 2)) {
+          const double term = !a.adjoint
+                                  ? coef * val * a.dt * ob[j] / (comp == 0 ? a.d4x : a.d4z)
+                                  : a.dt * ob[j] * val / a.dxdz;
+          if (comp == 0)
+            pvx[j] += term;
+          else
+            pvz[j] += term;
+        }
+    }
+    ptb = tb;
+    pz0 = z0;
+    pend = true;
+    if (k + 1 == nk) {
+      double *vxo = a.vx_o + tb, *vzo = a.vz_o + tb;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int r = w + 4 * j;
+        if (xin && z0 + r < a.nz) {
+          const unsigned o = off(r, lane);
+          stb(vxo, o, pvx[j]);
+          stb(vzo, o, pvz[j]);
+        }
+      }
+    }
+  }
+}
+
+// Block -> (strip, segment of P tiles); with xcd_remap, blocks that the
+// dispatcher sends to one XCD get a contiguous run of (strip, segment)
+// items, x fastest.
+template <int TZ, int P, int WPE, bool CORR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+step_lean_pipe(StepArgs a) {
+  const int nseg = (a.ntz + P - 1) / P;
+  const int nitems = a.ntx * nseg;
+  int b = blockIdx.x;
+  if (a.xcd_remap) b = (b % 8) * ((nitems + 7) / 8) + b / 8;
+  if (b >= nitems) return;
+  lean_pipe_strip<TZ, P, CORR>(a, b % a.ntx, b / a.ntx, a.it);
 }
 
 // ---------------------------------------------------------------------------
@@ -1608,6 +2032,14 @@ __device__ inline int batch_tile(const BatchLaunch &b) {
   return (blk % 8) * chunk + blk / 8;
 }
 
+// The snapshot slot of step b.it in a member's one-shot store.
+__device__ inline void batch_snapshot_slot(StepArgs &a, const BatchLaunch &b) {
+  a.do_snap = b.it % b.si == 0;
+  const long long off = (long long)(b.it / b.si) * a.nx * a.nz;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) a.snap[f] += off;
+}
+
 template <int TZ, int NT>
 __global__ void __launch_bounds__(NT) batch_fused(BatchLaunch b) {
   const int g = batch_tile(b);
@@ -1615,6 +2047,7 @@ __global__ void __launch_bounds__(NT) batch_fused(BatchLaunch b) {
   const int2 mt = b.tiles[g];
   StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
   a.it = b.it;
+  if (b.snapshots) batch_snapshot_slot(a, b);
   fused_tile<TZ, false, NT>(a, mt.y);
 }
 
@@ -1629,14 +2062,15 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
   two_step_tile<TZ, NT>(a, mt.y);
 }
 
-template <int TZ, int WPE>
+template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
   if (g >= b.ntiles) return;
   const int2 mt = b.tiles[g];
-  const StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
-  lean_tile<TZ, false>(a, mt.y, b.it);
+  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  if (b.snapshots) batch_snapshot_slot(a, b);
+  lean_tile<TZ, CORR, HALO_DMA>(a, mt.y, b.it);
 }
 
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
@@ -1952,12 +2386,17 @@ int step_tile_height(int variant) {
       return 24;
     case kLeanTz32:
       return 32;
+    case kLeanTz4:
+      return 4;
+    case kLeanTz12W5:
+      return 12;
     default:
       return 8;
   }
 }
 bool step_is_lean(int variant) {
-  return variant >= kLeanTz16 && variant <= kLeanTz32;
+  return (variant >= kLeanTz16 && variant <= kLeanTz32) ||
+         (variant >= kLeanTz4 && variant <= kLeanTz8Halo);
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -1979,7 +2418,10 @@ int step_single_variant(int variant) {
   return variant;
 }
 bool step_uses_fastdiv(int variant) {
-  return step_is_lean(variant) || variant == kLeanTwoTz8;
+  return step_is_lean(variant) || variant == kLeanTwoTz8 || step_is_lean_pipe(variant);
+}
+bool step_is_lean_pipe(int variant) {
+  return variant >= kLeanPipe2Tz8 && variant <= kLeanPipe4Tz8;
 }
 
 hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
@@ -1998,29 +2440,33 @@ hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t s,
 }
 
 hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
-                             hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                             hipStream_t s, hipEvent_t e0, hipEvent_t e1, bool corr) {
   int nblocks = b.xcd_remap ? 8 * ((b.ntiles + 7) / 8) : b.ntiles;
-  dim3 g(nblocks);
+  dim3 g(nblocks), t(256);
   const bool tall = step_tile_height(variant) == 16;
+  // one-step lean kernels; adjoint snapshot steps use the correlating
+  // instantiation (one wave per SIMD fewer, as for single models)
+#define PSV_BATCH_LEAN(TZ, W, WC)                                                  \
+  if (corr)                                                                        \
+    hipExtLaunchKernelGGL((batch_lean<TZ, WC, true>), g, t, 0, s, e0, e1, 0, b);   \
+  else                                                                             \
+    hipExtLaunchKernelGGL((batch_lean<TZ, W, false>), g, t, 0, s, e0, e1, 0, b);
   if (step_is_lean(variant)) {
     switch (variant) {
-      case kLeanTz16:
-        hipExtLaunchKernelGGL((batch_lean<16, 3>), g, dim3(256), 0, s, e0, e1, 0, b);
-        break;
-      case kLeanTz8:
-        hipExtLaunchKernelGGL((batch_lean<8, 5>), g, dim3(256), 0, s, e0, e1, 0, b);
-        break;
-      case kLeanTz12:
-        hipExtLaunchKernelGGL((batch_lean<12, 4>), g, dim3(256), 0, s, e0, e1, 0, b);
-        break;
-      case kLeanTz20:
-        hipExtLaunchKernelGGL((batch_lean<20, 3>), g, dim3(256), 0, s, e0, e1, 0, b);
-        break;
-      case kLeanTz24:
-        hipExtLaunchKernelGGL((batch_lean<24, 2>), g, dim3(256), 0, s, e0, e1, 0, b);
-        break;
-      case kLeanTz32:
-        hipExtLaunchKernelGGL((batch_lean<32, 2>), g, dim3(256), 0, s, e0, e1, 0, b);
+      case kLeanTz16: PSV_BATCH_LEAN(16, 3, 2) break;
+      case kLeanTz8: PSV_BATCH_LEAN(8, 5, 4) break;
+      case kLeanTz12: PSV_BATCH_LEAN(12, 4, 3) break;
+      case kLeanTz20: PSV_BATCH_LEAN(20, 3, 2) break;
+      case kLeanTz24: PSV_BATCH_LEAN(24, 2, 1) break;
+      case kLeanTz32: PSV_BATCH_LEAN(32, 2, 1) break;
+      case kLeanTz4: PSV_BATCH_LEAN(4, 7, 5) break;
+      case kLeanTz8W6: PSV_BATCH_LEAN(8, 6, 4) break;
+      case kLeanTz12W5: PSV_BATCH_LEAN(12, 5, 3) break;
+      case kLeanTz8Halo:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<8, 4, true, true>), g, t, 0, s, e0, e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<8, 6, false, true>), g, t, 0, s, e0, e1, 0, b);
         break;
       default:
         return hipErrorInvalidValue;
@@ -2029,13 +2475,14 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
     if (tall)
       hipExtLaunchKernelGGL((batch_two<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
     else
-      hipExtLaunchKernelGGL((batch_two<8, 256>), g, dim3(256), 0, s, e0, e1, 0, b);
+      hipExtLaunchKernelGGL((batch_two<8, 256>), g, t, 0, s, e0, e1, 0, b);
   } else {
     if (tall)
       hipExtLaunchKernelGGL((batch_fused<16, 512>), g, dim3(512), 0, s, e0, e1, 0, b);
     else
-      hipExtLaunchKernelGGL((batch_fused<8, 256>), g, dim3(256), 0, s, e0, e1, 0, b);
+      hipExtLaunchKernelGGL((batch_fused<8, 256>), g, t, 0, s, e0, e1, 0, b);
   }
+#undef PSV_BATCH_LEAN
   return hipGetLastError();
 }
 
@@ -2115,6 +2562,54 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
       else
         hipExtLaunchKernelGGL((step_lean<32, 2, false>), g, b, 0, s, e0, e1, 0, a);
       break;
+    case kLeanTz4:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<4, 5, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<4, 7, false>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz8W6:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 4, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 6, false>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz12W5:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<12, 3, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<12, 5, false>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz8Halo:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 4, true, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 6, false, true>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanPipe2Tz8:
+    case kLeanPipe3Tz8:
+    case kLeanPipe4Tz8: {
+      const int P = variant == kLeanPipe2Tz8 ? 2 : variant == kLeanPipe3Tz8 ? 3 : 4;
+      const int items = a.ntx * ((a.ntz + P - 1) / P);
+      dim3 gp(a.xcd_remap ? 8 * ((items + 7) / 8) : items);
+      if (P == 2) {
+        if (corr)
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 2, 3, true>), gp, b, 0, s, e0, e1, 0, a);
+        else
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 2, 4, false>), gp, b, 0, s, e0, e1, 0, a);
+      } else if (P == 3) {
+        if (corr)
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 3, 3, true>), gp, b, 0, s, e0, e1, 0, a);
+        else
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 3, 4, false>), gp, b, 0, s, e0, e1, 0, a);
+      } else {
+        if (corr)
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 4, 3, true>), gp, b, 0, s, e0, e1, 0, a);
+        else
+          hipExtLaunchKernelGGL((step_lean_pipe<8, 4, 4, false>), gp, b, 0, s, e0, e1, 0, a);
+      }
+      break;
+    }
     default:
       return hipErrorInvalidValue;
   }

@@ -40,6 +40,21 @@ class Model {
   void map_kernels_to_velocity();
   void run_model(bool verbose, bool simulate_adjoint);
 
+  // Asynchronous per-shot calls (on by default): forward() and adjoint()
+  // enqueue their time loops and return.  Forwards alternate between two
+  // lanes (HIP streams with their own state buffers); adjoints pair up --
+  // the first of a pair updates the kernels, the second runs beside it on
+  // the other lane with its kernel terms deferred to a buffer and added
+  // afterwards in its own order -- so every result is bitwise that of the
+  // synchronous calls.  settle() waits for the enqueued loops (and adds
+  // pending deferred terms); every other method must be called on a settled
+  // model -- the C-ABI settles before each of them.  Verbose calls, timing,
+  // graph mode, checkpointing and the two-step and in-place variants run
+  // synchronously.
+  void set_async(bool on);
+  bool async() const { return async_; }
+  void settle();
+
   // parameters and data, host buffers in the reference layout
   void get_parameter_fields(double *vp, double *vs, double *rho) const;
   void set_parameter_fields(const double *vp, const double *vs,
@@ -89,6 +104,12 @@ class Model {
   // (psv_batch.hip): the arguments of a launch that reads state buffer set
   // `parity`, zeroing the state, and the receiver scan of two-step loops.
   StepArgs forward_launch_args(int i_shot, int parity) const;
+  // Batched gradients: the arguments of an adjoint launch reading state set
+  // `parity` (adjoint sources of shot i_shot, kernel correlation), and a
+  // snapshot store for one shot at a time ([slot][iz][ix] per field) whose
+  // bases both launch kinds get in StepArgs::snap.
+  StepArgs adjoint_launch_args(int i_shot, int parity) const;
+  double *const *shot_snapshot_store();
   void enqueue_zero_state(hipStream_t s) const;
   void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
   int tile_count() const { return ntiles_; }
@@ -135,6 +156,7 @@ class Model {
   double taper_of_count(int c) const;
   double *syn_[2] = {}, *obs_[2] = {}, *adj_[2] = {};
   double *snap_[5] = {};
+  double *shot_snap_[5] = {};  // one shot's snapshots (batched gradients)
   long long *rec_idx_ = nullptr;
   int *fwd_off_ = nullptr, *adj_off_ = nullptr;
   InjEntry *fwd_entries_ = nullptr, *adj_entries_ = nullptr;
@@ -185,6 +207,23 @@ class Model {
   int seg_k_ = 0;              // slots per segment buffer in seg_
   std::vector<hipEvent_t> ckpt_events_;
   int *scratch_off_ = nullptr; // all-zero CSR offsets
+
+  // asynchronous per-shot calls (see set_async)
+  bool async_ = true;
+  bool lane_busy_[2] = {false, false};  // work enqueued since the last settle
+  std::vector<int> fwd_pending_;        // per shot: 0, or 1 + lane of its forward
+  std::vector<hipEvent_t> fwd_events_;  // end of each shot's pending forward
+  int fwd_calls_ = 0, adj_calls_ = 0;   // since the last settle
+  bool defer_pending_ = false;          // lane Y's adjoint terms not yet added
+  bool replay_enqueued_ = false;        // ev_replayed_ marks the last replay
+  bool no_defer_room_ = false;          // the device had no room for defer_
+  hipEvent_t ev_defer_done_ = nullptr, ev_replayed_ = nullptr, ev_join_ = nullptr;
+  bool async_path(bool verbose, bool dump) const;
+  void ensure_lane();
+  void wait_forward(int i_shot, int lane);
+  void enqueue_replay(hipStream_t s);
+  void enqueue_loop_on(StepArgs &a, int i_shot, bool forward, bool store, hipStream_t s,
+                       double *(*st)[5], double *defer);
 };
 
 }  // namespace psv

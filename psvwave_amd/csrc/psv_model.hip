@@ -77,6 +77,7 @@ Model::Model(const Model &o) : cfg_(o.cfg_), d_(o.d_) {
     cp(adj_[k], o.adj_[k], nrec_);
   }
   ckpt_every_ = o.ckpt_every_;
+  async_ = o.async_;
   if (o.snap_[0]) {
     ensure_snapshots();
     long long n = (long long)cfg_.n_shots * stored_snapshots() * ngrid_;
@@ -93,6 +94,9 @@ Model::~Model() {
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : lane_events_) (void)hipEventDestroy(e);
   for (auto e : ckpt_events_) (void)hipEventDestroy(e);
+  for (auto e : fwd_events_) (void)hipEventDestroy(e);
+  for (auto e : {ev_defer_done_, ev_replayed_, ev_join_})
+    if (e) (void)hipEventDestroy(e);
   if (lane_stream_) (void)hipStreamDestroy(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
@@ -353,7 +357,7 @@ void Model::set_step_variant(int variant, int xcd_remap) {
   if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
   if (step_uses_fastdiv(variant) && !lean_ok_)
-    throw ValueError("step variants 8-14 need dx and dz in [2^-40, 2^40] "
+    throw ValueError("step variants 8-20 need dx and dz in [2^-40, 2^40] "
                      "and a taper that grows towards the interior");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
@@ -607,6 +611,104 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
               "receiver scan");
 }
 
+// One whole time loop on stream s with state buffers st (the prologue of
+// run_model_two_lanes' jobs): the adjoint source of an adjoint's shot is in
+// place already; with `defer` the adjoint's kernel terms go to that zeroed
+// per-slot buffer instead of the kernels.
+void Model::enqueue_loop_on(StepArgs &a, int i_shot, bool forward, bool store,
+                            hipStream_t s, double *(*st)[5], double *defer) {
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 5; ++f)
+      hip_check(launch_fill(st[k][f], npad_, 0.0, s), "zero state");
+  if (defer) hip_check(launch_fill(defer_, defer_n_, 0.0, s), "zero deferred terms");
+  a.rec_increments = 0;
+  a.rec_off = adj_off_;
+  a.rec_entries = adj_entries_;
+  SlotFn slot = [this, i_shot](int it, double **dst) {
+    return stored_slot(i_shot, it, dst);
+  };
+  enqueue_range(a, st, 0, forward, 0, cfg_.nt, store || !forward, slot, nullptr, false, s,
+                defer);
+}
+
+// ---- asynchronous per-shot calls ----
+
+void Model::set_async(bool on) {
+  settle();
+  async_ = on;
+}
+
+// The asynchronous path covers one-step, ping-pong kernels without timing,
+// graphs, checkpointing, verbose output or wavefield dumps.
+bool Model::async_path(bool verbose, bool dump) const {
+  return async_ && !verbose && !dump && timing_stride_ == 0 && !graph_mode_ &&
+         ckpt_every_ == 1 && !step_is_two_step(variant_) && !step_is_inplace(variant_);
+}
+
+// Lane Y: its stream, state buffers and the per-shot completion events.
+// Allocation zero-fills on the model stream, so the first use waits for it.
+void Model::ensure_lane() {
+  bool fresh = false;
+  if (!lane_stream_)
+    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
+              "hipStreamCreate");
+  if (!lane_state_[0][0]) {
+    for (int p = 0; p < 2; ++p)
+      for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
+    fresh = true;
+  }
+  while ((int)fwd_events_.size() < cfg_.n_shots) {
+    hipEvent_t e;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+    fwd_events_.push_back(e);
+  }
+  for (hipEvent_t *e : {&ev_defer_done_, &ev_replayed_, &ev_join_})
+    if (!*e) hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+  if ((int)fwd_pending_.size() < cfg_.n_shots) fwd_pending_.assign(cfg_.n_shots, 0);
+  if (fresh) sync();
+}
+
+// Lane `lane` (0 = X, the model stream; 1 = Y) waits for shot i_shot's
+// pending forward when that ran on the other lane.
+void Model::wait_forward(int i_shot, int lane) {
+  const int f = fwd_pending_[i_shot];
+  if (f && f - 1 != lane)
+    hip_check(hipStreamWaitEvent(lane ? lane_stream_ : stream_, fwd_events_[i_shot], 0),
+              "hipStreamWaitEvent");
+}
+
+// Adds lane Y's deferred adjoint terms to the kernels on stream s (after
+// that adjoint has finished), in the adjoint's own order.
+void Model::enqueue_replay(hipStream_t s) {
+  hip_check(hipStreamWaitEvent(s, ev_defer_done_, 0), "hipStreamWaitEvent");
+  hip_check(launch_replay_correlation(
+                rho_l_, lam_k_, mu_k_, defer_, d_.snapshots, ngrid_, d_.nx,
+                cfg_.np_boundary + cfg_.nx_inner_boundary,
+                cfg_.np_boundary + cfg_.nx_inner - cfg_.nx_inner_boundary,
+                cfg_.np_boundary + cfg_.nz_inner_boundary,
+                cfg_.np_boundary + cfg_.nz_inner - cfg_.nz_inner_boundary, s),
+            "replay correlation");
+  hip_check(hipEventRecord(ev_replayed_, s), "hipEventRecord");
+  defer_pending_ = false;
+  replay_enqueued_ = true;
+}
+
+// Waits for every enqueued per-shot loop and adds pending deferred terms.
+// Errors of the enqueued work surface here.
+void Model::settle() {
+  if (!lane_busy_[0] && !lane_busy_[1] && !defer_pending_) return;
+  if (defer_pending_) enqueue_replay(stream_);
+  if (lane_busy_[1]) {
+    hip_check(hipEventRecord(ev_join_, lane_stream_), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(stream_, ev_join_, 0), "hipStreamWaitEvent");
+  }
+  lane_busy_[0] = lane_busy_[1] = false;
+  std::fill(fwd_pending_.begin(), fwd_pending_.end(), 0);
+  fwd_calls_ = adj_calls_ = 0;
+  replay_enqueued_ = false;
+  sync();
+}
+
 // Adjoint with checkpointing: segments of ckpt_every_ snapshot intervals
 // are processed last to first.  For each, the forward state is restored
 // from the segment's checkpoint into a second buffer pair, the segment's
@@ -818,6 +920,35 @@ StepArgs Model::forward_launch_args(int i_shot, int parity) const {
   return a;
 }
 
+StepArgs Model::adjoint_launch_args(int i_shot, int parity) const {
+  check_shot(i_shot);
+  StepArgs a = adjoint_args(i_shot);
+  a.rec_off = adj_off_;
+  a.rec_entries = adj_entries_;
+  const int cur = parity & 1, nxt = 1 - cur;
+  a.vx_i = state_[cur][0];
+  a.vz_i = state_[cur][1];
+  a.txx_i = state_[cur][2];
+  a.tzz_i = state_[cur][3];
+  a.txz_i = state_[cur][4];
+  a.vx_o = state_[nxt][0];
+  a.vz_o = state_[nxt][1];
+  a.txx_o = state_[nxt][2];
+  a.tzz_o = state_[nxt][3];
+  a.txz_o = state_[nxt][4];
+  a.do_snap = 0;
+  a.do_snap1 = 0;
+  return a;
+}
+
+double *const *Model::shot_snapshot_store() {
+  if (!shot_snap_[0]) {
+    for (int f = 0; f < 5; ++f) shot_snap_[f] = dalloc((size_t)d_.snapshots * ngrid_);
+    sync();
+  }
+  return shot_snap_;
+}
+
 void Model::enqueue_zero_state(hipStream_t s) const {
   for (int k = 0; k < 2; ++k)
     for (int f = 0; f < 5; ++f)
@@ -837,6 +968,28 @@ void Model::forward(int i_shot, bool store_fields, bool verbose,
   if (store_fields && cfg_.nt % cfg_.snapshot_interval != 0)
     throw ValueError("Snapshot interval and size of accumulator don't match!"
                      " (nt must be a multiple of snapshot_interval)");
+  if (async_path(verbose, output_wavefields)) {
+    ensure_lane();
+    // a pending adjoint may still read this shot's snapshots and a pending
+    // forward of the same shot writes the same traces: wait for them; a new
+    // snapshot store is zero-filled on the model stream before any lane
+    // writes to it
+    if (adj_calls_ > 0 || fwd_pending_[i_shot] || (store_fields && !snap_[0])) settle();
+    if (store_fields && !snap_[0]) {
+      ensure_snapshots();
+      sync();
+    }
+    const int lane = fwd_calls_ & 1;
+    hipStream_t s = lane ? lane_stream_ : stream_;
+    StepArgs a = forward_args(i_shot);
+    enqueue_loop_on(a, i_shot, true, store_fields, s, lane ? lane_state_ : state_, nullptr);
+    hip_check(hipEventRecord(fwd_events_[i_shot], s), "hipEventRecord");
+    fwd_pending_[i_shot] = 1 + lane;
+    lane_busy_[lane] = true;
+    ++fwd_calls_;
+    return;
+  }
+  settle();
   if (store_fields) ensure_snapshots();
   StepArgs a = forward_args(i_shot);
   auto t0 = std::chrono::steady_clock::now();
@@ -857,6 +1010,36 @@ void Model::adjoint(int i_shot, bool verbose) {
   if (!snap_[0])
     throw ValueError("adjoint_simulate needs stored forward wavefields; run "
                      "forward_simulate with store_fields=True first");
+  if (async_path(verbose, false)) {
+    ensure_lane();
+    if (!defer_ && !no_defer_room_) {
+      // a new buffer is zero-filled on the model stream: allocate it on a
+      // settled model and let the fill finish before a lane writes to it
+      settle();
+      no_defer_room_ = !ensure_defer_buffer();
+      sync();
+    }
+    const int lane = defer_ ? (adj_calls_ & 1) : 0;
+    StepArgs a = adjoint_args(i_shot);
+    if (lane == 0) {
+      // the previous adjoint's deferred terms go into the kernels first
+      if (defer_pending_) enqueue_replay(stream_);
+      wait_forward(i_shot, 0);
+      enqueue_loop_on(a, i_shot, false, false, stream_, state_, nullptr);
+    } else {
+      // the deferred-term buffer is free once the last replay has read it
+      if (replay_enqueued_)
+        hip_check(hipStreamWaitEvent(lane_stream_, ev_replayed_, 0), "hipStreamWaitEvent");
+      wait_forward(i_shot, 1);
+      enqueue_loop_on(a, i_shot, false, false, lane_stream_, lane_state_, defer_);
+      hip_check(hipEventRecord(ev_defer_done_, lane_stream_), "hipEventRecord");
+      defer_pending_ = true;
+    }
+    lane_busy_[lane] = true;
+    ++adj_calls_;
+    return;
+  }
+  settle();
   StepArgs a = adjoint_args(i_shot);
   auto t0 = std::chrono::steady_clock::now();
   if (ckpt_every_ > 1 && timing_stride_ < 0) {

@@ -163,7 +163,9 @@ class fdModel:
         """Run the forward simulation of shot `i_shot` (src/fdModel.cpp:526-787).
 
         `omp_threads_override` is accepted for signature compatibility; the
-        simulation runs on the GPU and does not use host threads.
+        simulation runs on the GPU and does not use host threads.  The call
+        returns once the time loop is enqueued (see set_async); every other
+        method waits for it.
         """
         self._generation += 1
         _lib.check(self._lib.psv_forward(self._h, int(i_shot),
@@ -173,7 +175,8 @@ class fdModel:
 
     def adjoint_simulate(self, i_shot, verbose=False, omp_threads_override=0):
         """Adjoint simulation of shot `i_shot`, correlating with the stored
-        forward snapshots into the Lame kernels (src/fdModel.cpp:789-969)."""
+        forward snapshots into the Lame kernels (src/fdModel.cpp:789-969).
+        Returns once the time loop is enqueued (see set_async)."""
         _lib.check(self._lib.psv_adjoint(self._h, int(i_shot),
                                          int(bool(verbose))))
 
@@ -390,6 +393,21 @@ class fdModel:
     def set_graph_mode(self, on=True):
         """Replay each shot's time loop as a captured HIP graph."""
         _lib.check(self._lib.psv_set_graph_mode(self._h, int(bool(on))))
+
+    def set_async(self, on):
+        """on (the default): forward_simulate and adjoint_simulate return as
+        soon as their time loop is enqueued on the GPU -- forwards of
+        consecutive calls run two at a time, adjoints in pairs with the
+        second one's kernel terms added afterwards in its own order -- and
+        every other method waits for them first; results are bitwise those
+        of synchronous calls.  off: each call returns when its loop has
+        finished, as in the reference."""
+        _lib.check(self._lib.psv_set_async(self._h, int(bool(on))))
+
+    def synchronize(self):
+        """Wait for the time loops enqueued by forward_simulate and
+        adjoint_simulate."""
+        _lib.check(self._lib.psv_synchronize(self._h))
 
     def set_checkpointing(self, every):
         """Keep every `every`-th forward snapshot only and recompute the

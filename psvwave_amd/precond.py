@@ -7,8 +7,9 @@ the surface.  Models are a smooth Gaussian-random-field background plus a
 +-5 % random perturbation inside the kernel region.
 
 * ``gradient_pairs``: (gradient at the background, true perturbation) pairs,
-  computed by several device models at once (one host thread and one HIP
-  stream each; the C-ABI releases the GIL).
+  computed for many models at once by the batched solver (``Batch``: one
+  launch per time step over every member, forward and adjoint with kernel
+  correlation).
 * ``UNet``: a 4-level U-Net (32 base channels) mapping the normalised
   velocity kernels to a relative model update.  ``train_hip`` /
   ``HipUNetTrainer`` train it and ``HipUNet`` runs it on this library's
@@ -23,7 +24,6 @@ The gradient is the reference's (``get_kernels`` after the adjoint,
 src/fdModel.cpp:789-969, 1164-1183); the FWI loop follows the descent loop of
 tests/test_descent.py:106-129 with a line search added.
 """
-import threading
 import time
 
 import numpy as np
@@ -116,55 +116,61 @@ def misfit_at(model, fields):
     return model.misfit
 
 
-def gradient_pairs(n, seed=7, workers=8, cfg=None, log=None, along_path=0.0):
+def gradient_pairs(n, seed=7, batch_size=256, cfg=None, log=None, along_path=0.0):
     """n (gradient, perturbation) pairs as float32 arrays of shape
     (n, 3, nx, nz): the velocity kernels at the background model against
     data of the true model, and (true - background) / background.  With
     along_path = f > 0, a fraction f of the pairs is taken instead at a
     model part of the way to the true one, m = bg + t (true - bg) with
     t ~ U(0.2, 0.8): the kernels there and the remaining (true - m) / m --
-    what the network sees in later FWI iterations."""
-    from .model import fdModel
+    what the network sees in later FWI iterations.
+
+    The device solver runs `batch_size` pairs at once as one Batch: the
+    true models' forward runs give the observed data, then Batch.gradients
+    computes every member's gradient (bitwise each model's run_model) with
+    one launch per time step over all members.  The next batch's models are
+    drawn on a host thread meanwhile.  Pair k depends only on (seed, k)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from .batch import Batch
     cfg = cfg or c5_config()
-    rngs = [np.random.default_rng([seed, k]) for k in range(n)]
-    out_g = [None] * n
-    out_p = [None] * n
-    next_k = [0]
-    lock = threading.Lock()
-    errors = []
 
-    def work():
-        try:
-            m = fdModel(cfg)
-            while True:
-                with lock:
-                    k = next_k[0]
-                    next_k[0] += 1
-                    if log and k < n and k % 128 == 0:
-                        log(f"training pair {k}/{n}")
-                if k >= n:
-                    return
-                bg, true = c5_models(rngs[k])
-                if rngs[k].random() < along_path:
-                    t = rngs[k].uniform(0.2, 0.8)
-                    bg = tuple(b + t * (tr - b) for b, tr in zip(bg, true))
-                _simulate(m, true, store=False)
-                m.set_observed_data(*m.get_synthetic_data())
-                m.set_parameter_fields(*bg)
-                g = gradient(m, two_lanes=False)  # eight models already share the GPU
-                out_g[k] = np.stack(g).astype(np.float32)
-                out_p[k] = np.stack([(t - b) / b for t, b in zip(true, bg)]).astype(np.float32)
-        except Exception as e:  # surfaced in the caller
-            errors.append(e)
+    def draw(k):
+        rng = np.random.default_rng([seed, k])
+        bg, true = c5_models(rng)
+        if rng.random() < along_path:
+            t = rng.uniform(0.2, 0.8)
+            bg = tuple(b + t * (tr - b) for b, tr in zip(bg, true))
+        return bg, true
 
-    threads = [threading.Thread(target=work) for _ in range(max(1, workers))]
-    for t in threads:
-        t.start()
-    for t in threads:
-        t.join()
-    if errors:
-        raise errors[0]
-    return np.stack(out_g), np.stack(out_p)
+    def draw_all(ks):
+        return [draw(k) for k in ks]
+
+    size = max(1, min(batch_size, n))
+    batch = Batch([cfg] * size)
+    shape = (batch.infos[0]["nx"], batch.infos[0]["nz"])
+    out_g = np.empty((n, 3) + shape, dtype=np.float32)
+    out_p = np.empty((n, 3) + shape, dtype=np.float32)
+    starts = list(range(0, n, size))
+    with ThreadPoolExecutor(1) as pool:
+        nxt = pool.submit(draw_all, range(0, min(n, size)))
+        for i, k0 in enumerate(starts):
+            models = nxt.result()
+            if i + 1 < len(starts):
+                k1 = starts[i + 1]
+                nxt = pool.submit(draw_all, range(k1, min(n, k1 + size)))
+            for j, (_, true) in enumerate(models):
+                batch.set_parameter_fields(j, *true)
+            batch.forward_all()
+            for j, (bg, _) in enumerate(models):
+                batch.set_observed_data(j, *batch.get_synthetic_data(j))
+                batch.set_parameter_fields(j, *bg)
+            batch.gradients()
+            for j, (bg, true) in enumerate(models):
+                out_g[k0 + j] = np.stack(batch.get_kernels(j))
+                out_p[k0 + j] = np.stack([(t - b) / b for t, b in zip(true, bg)])
+            if log:
+                log(f"training pairs {k0 + len(models)}/{n}")
+    return out_g, out_p
 
 
 def normalise_gradient(g):
@@ -188,13 +194,14 @@ def _unet_class():
                              nn.Conv2d(co, co, 3, padding=1), nn.GELU())
 
     class UNet(nn.Module):
-        """4-level U-Net, 32 base channels; input and output 3 channels
-        (vp, vs, rho) on the full (nx, nz) grid, zero-padded to a multiple
-        of 16."""
+        """4-level U-Net (two 3 x 3 convolutions per level, channel widths
+        `widths`, by default 32, 48, 64, 128: 0.58 M parameters); input and
+        output 3 channels (vp, vs, rho) on the full (nx, nz) grid,
+        zero-padded to a multiple of 16."""
 
-        def __init__(self, base=32):
+        def __init__(self, widths=(32, 48, 64, 128)):
             super().__init__()
-            c = [base, 2 * base, 4 * base, 8 * base]
+            c = list(widths)
             self.enc = nn.ModuleList([block(3, c[0]), block(c[0], c[1]), block(c[1], c[2])])
             self.mid = block(c[2], c[3])
             self.up = nn.ModuleList([nn.ConvTranspose2d(c[3], c[2], 2, stride=2),
@@ -221,8 +228,16 @@ def _unet_class():
     return UNet
 
 
-def make_unet(base=32):
-    return _unet_class()(base)
+def make_unet(widths=(32, 48, 64, 128), base=None):
+    """The preconditioner network with channel widths `widths` (four
+    levels), or (base, 2 base, 4 base, 8 base) when `base` is given --
+    base = 32 is the wider 1.9 M-parameter variant.  The library's
+    convolution kernels need widths that are multiples of 8."""
+    if base is not None:
+        widths = (base, 2 * base, 4 * base, 8 * base)
+    if len(widths) != 4:
+        raise ValueError("four channel widths")
+    return _unet_class()(tuple(int(c) for c in widths))
 
 
 def fwi(model, start, iterations=30, direction=None, step0=0.01, max_halvings=6,

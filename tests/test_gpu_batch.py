@@ -35,7 +35,8 @@ def _fields(nx, nz, seed):
 @pytest.mark.parametrize("variant,graph", [(0, False), (1, False), (6, False),
                                            (7, False), (6, True), (8, False),
                                            (9, False), (8, True), (10, False),
-                                           (11, False), (12, False), (13, False)])
+                                           (11, False), (12, False), (13, False),
+                                           (18, False), (19, False), (20, False)])
 def test_batch_members_equal_single_models(oracle_lib, variant, graph):
     import psvwave_amd
     cfgs = _members()
@@ -78,11 +79,64 @@ def test_batch_autotune_and_validation():
     cfgs = _members()
     batch = psvwave_amd.Batch(cfgs[:2])
     best, times = batch.autotune(steps=6)
-    assert best in (0, 1, 6, 7, 8, 9, 10, 11, 12, 13)
-    assert set(times) == {0, 1, 6, 7, 8, 9, 10, 11, 12, 13}
+    tuned = {0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20}
+    assert best in tuned
+    assert set(times) == tuned
     assert all(t > 0 for t in times.values())
     bad = small_config(**{"domain.nt": 50})
     with pytest.raises(ValueError, match="share nt"):
         psvwave_amd.Batch([cfgs[0], bad])
     with pytest.raises(ValueError, match="variants"):
         batch.set_variant(2)
+
+
+@pytest.mark.parametrize("variant", [9, 0, 18])
+def test_batch_gradients_equal_single_models(oracle_lib, variant):
+    """Batched FWI gradients (forward with snapshots, adjoint with kernel
+    correlation, shot by shot over all members) equal each member's own
+    run_model bit for bit, and one member equals the oracle."""
+    import psvwave_amd
+    cfgs = _members()
+    for c in cfgs:
+        c["domain"]["nt"] = 100
+    batch = psvwave_amd.Batch(cfgs)
+    batch.set_variant(variant)
+    singles, starts = [], []
+    for m, cfg in enumerate(cfgs):
+        info = batch.infos[m]
+        true = _fields(info["nx"], info["nz"], m)
+        start = _fields(info["nx"], info["nz"], 100 + m)
+        batch.set_parameter_fields(m, *true)
+        starts.append(start)
+    batch.forward_all()
+    for m, cfg in enumerate(cfgs):
+        obs = batch.get_synthetic_data(m)
+        batch.set_observed_data(m, *obs)
+        batch.set_parameter_fields(m, *starts[m])
+        single = psvwave_amd.fdModel(cfg)
+        single.set_observed_data(*obs)
+        single.set_parameter_fields(*starts[m])
+        singles.append((single, obs))
+    for rep in range(2):
+        misfits = batch.gradients()
+        for m, (single, _) in enumerate(singles):
+            if rep == 0:
+                single.run_model(False, True)
+            assert misfits[m] == single.misfit, (variant, m)
+            for a, b in zip(batch.get_kernels(m), single.get_kernels()):
+                assert np.array_equal(a, b), (variant, m, rep)
+            assert any(np.abs(k).max() > 0 for k in single.get_kernels())
+    o = oracle_lib.OracleModel(cfgs[3])
+    o.set_parameter_fields(*starts[3])
+    o.set_observed_data(*singles[3][1])
+    for s in range(2):
+        o.forward_simulate(s)
+    o.calculate_l2_misfit()
+    o.calculate_l2_adjoint_sources()
+    o.reset_kernels()
+    for s in range(2):
+        o.adjoint_simulate(s)
+    o.map_kernels_to_velocity()
+    for a, b in zip(batch.get_kernels(3), o.get_kernels()):
+        assert np.array_equal(a, b)
+    assert abs(misfits[3] - o.misfit) <= 1e-12 * o.misfit

@@ -26,8 +26,20 @@ def test_precond_pipeline_small():
     import psvwave_amd
     from psvwave_amd import precond
     cfg = precond.c5_config(nt=600)
-    g, p = precond.gradient_pairs(6, seed=1, workers=3, cfg=cfg)
+    g, p = precond.gradient_pairs(6, seed=1, batch_size=4, cfg=cfg)
     assert g.shape == (6, 3, 316, 188) and p.shape == g.shape
+    # pair 5 (second batch) is the gradient one model's run_model gives
+    rng = np.random.default_rng([1, 5])
+    bg, true = precond.c5_models(rng)
+    one = psvwave_amd.fdModel(cfg)
+    one.set_parameter_fields(*true)
+    for s in range(one.n_shots):
+        one.forward_simulate(s, store_fields=False)
+    one.set_observed_data(*one.get_synthetic_data())
+    one.set_parameter_fields(*bg)
+    one.run_model(False, True)
+    assert np.array_equal(g[5], np.stack(one.get_kernels()).astype(np.float32))
+    del one
     assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
     mask = precond.kernel_mask((316, 188))
     assert np.all(p[:, :, mask == 0] == 0)
@@ -35,7 +47,7 @@ def test_precond_pipeline_small():
     assert np.all(g[:, :, mask == 0] == 0)
     # pairs part of the way to the true model: same models, smaller remaining
     # perturbations, different gradients
-    g2, p2 = precond.gradient_pairs(2, seed=1, workers=2, cfg=cfg, along_path=1.0)
+    g2, p2 = precond.gradient_pairs(2, seed=1, cfg=cfg, along_path=1.0)
     for k in range(2):
         assert np.abs(p2[k]).max() < np.abs(p[k]).max()
         assert np.all(np.sign(p2[k]) == np.sign(p[k]))
