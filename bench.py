@@ -123,6 +123,9 @@ def _args():
                         "gradient")
     p.add_argument("--c4-every", type=int, default=8)
     p.add_argument("--mc-models", type=int, default=512)
+    p.add_argument("--persistent", action="store_true",
+                   help="mc: run each shot's forward loop as one cooperative "
+                        "launch with a grid barrier between steps (variant 9)")
     return p.parse_args()
 
 
@@ -238,7 +241,12 @@ def monte_carlo(a):
         batch.set_parameter_fields(m, np.full(shape, med["scalar_vp"]),
                                    np.full(shape, med["scalar_vs"]),
                                    np.full(shape, med["scalar_rho"]))
-    best, tuned = batch.autotune(steps=20)
+    if a.persistent:
+        batch.set_variant(9)
+        batch.set_persistent(True)
+        best, tuned = 9, None
+    else:
+        best, tuned = batch.autotune(steps=20)
     if a.graph:
         batch.set_graph_mode(True)
     batch.forward_all()
@@ -280,6 +288,7 @@ def monte_carlo(a):
                    "grid_points": points, "nt": nt, "shots": 2, "receivers": 32,
                    "point_updates_per_step": upd, "variant": best,
                    "autotune_ms_per_step": tuned, "hip_graph": a.graph,
+                   "persistent": a.persistent,
                    "setup_s": round(setup, 1)},
         "roofline": {"bound": "hbm", "kernel": f"batch step (variant {best})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -294,6 +294,10 @@ int psv_batch_gradients(psv_batch *b);
 int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
                           double *rho_k);
 int psv_batch_set_graph_mode(psv_batch *b, int on);
+/* on = 1: each shot's forward time loop is one cooperative launch whose
+ * resident blocks meet at a grid barrier between steps, instead of one
+ * launch per step (step variant 9 only; the same results). */
+int psv_batch_set_persistent(psv_batch *b, int on);
 /* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20 (ms_per_step[13],
  * -1 for a variant the batch cannot use) and keeps the fastest (*best).
  * Overwrites shot 0's synthetic data. */

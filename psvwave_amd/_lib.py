@@ -127,6 +127,7 @@ _SIGNATURES = {
     "psv_batch_forward": [_P, _i],
     "psv_batch_misfits": [_P, _D],
     "psv_batch_gradients": [_P],
+    "psv_batch_set_persistent": [_P, _i],
     "psv_batch_get_kernels": [_P, _i, _D, _D, _D],
     "psv_batch_set_variant": [_P, _i, _i],
     "psv_batch_set_graph_mode": [_P, _i],

@@ -107,6 +107,12 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_variant(self._h, int(variant),
                                                    int(bool(xcd_remap))))
 
+    def set_persistent(self, on=True):
+        """One cooperative launch per shot's forward loop (blocks stay
+        resident, a grid barrier between steps) instead of one launch per
+        step; step variant 9 only."""
+        _lib.check(self._lib.psv_batch_set_persistent(self._h, int(bool(on))))
+
     def set_graph_mode(self, on=True):
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 

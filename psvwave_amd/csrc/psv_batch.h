@@ -32,6 +32,10 @@ class Batch {
   void set_variant(int variant, int xcd_remap);
   int variant() const { return variant_; }
   void set_graph_mode(bool on);
+  // One cooperative launch per shot's whole time loop (variant 9 only): the
+  // blocks stay resident and meet at a grid barrier between steps, instead
+  // of one launch per step.
+  void set_persistent(bool on);
   // Forward simulation of shot i_shot for every member (no snapshots).
   void forward(int i_shot);
   // Per-member L2 misfit over all shots (calculate_l2_misfit of each).
@@ -67,6 +71,7 @@ class Batch {
   int ntiles_ = 0;
   long long points_ = 0;
   bool graph_mode_ = false;
+  bool persistent_ = false;
   std::vector<hipGraphExec_t> graphs_;  // per shot
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double loop_ms_ = 0.0;

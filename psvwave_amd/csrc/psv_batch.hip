@@ -67,6 +67,13 @@ void Batch::clear_graphs() {
   graphs_.clear();
 }
 
+void Batch::set_persistent(bool on) {
+  if (on && variant_ != kLeanTz8)
+    throw ValueError("the persistent batch loop runs step variant 9 (lean 64 x 8)");
+  persistent_ = on;
+  clear_graphs();
+}
+
 void Batch::set_graph_mode(bool on) {
   if (!on) clear_graphs();
   graph_mode_ = on;
@@ -88,6 +95,7 @@ void Batch::set_variant(int variant, int xcd_remap) {
         throw ValueError("a member of the batch cannot use the lean step variants");
   variant_ = variant;
   xcd_remap_ = xcd_remap ? 1 : 0;
+  if (variant != kLeanTz8) persistent_ = false;
   clear_graphs();
   clear_gradient_tables();  // descriptors hold injection-table pointers
   build_tables();
@@ -128,6 +136,11 @@ void Batch::build_tables() {
 // running sums when the two-step kernels recorded increments.
 void Batch::enqueue_loop(int i_shot, int nsteps) {
   for (auto &m : models_) m->enqueue_zero_state(stream_);
+  if (persistent_) {
+    BatchLaunch b{descs_, tiles_, ntiles_, n_shots_, i_shot, 0, 0, 0, xcd_remap_, 0, 1};
+    hip_check(launch_batch_persistent(b, 0, nsteps, stream_), "persistent batch loop");
+    return;
+  }
   const bool two = step_is_two_step(variant_);
   int launch = 0;
   for (int k = 0; k < nsteps;) {

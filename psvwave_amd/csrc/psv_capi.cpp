@@ -737,6 +737,10 @@ int psv_batch_misfits(psv_batch *b, double *out) {
   });
 }
 
+int psv_batch_set_persistent(psv_batch *b, int on) {
+  return guard([&] { B(b).set_persistent(on != 0); });
+}
+
 int psv_batch_gradients(psv_batch *b) {
   return guard([&] { B(b).gradients(); });
 }

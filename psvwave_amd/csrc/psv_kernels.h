@@ -247,6 +247,10 @@ struct BatchLaunch {
 hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
                              hipStream_t stream, hipEvent_t e0 = nullptr,
                              hipEvent_t e1 = nullptr, bool corr = false);
+// Steps [k0, k1) of shot b.shot of every member in one cooperative launch
+// (lean 64 x 8 tiles; a grid barrier between steps).
+hipError_t launch_batch_persistent(const BatchLaunch &b, int k0, int k1,
+                                   hipStream_t stream);
 // bfloat16 MFMA kernels of the preconditioner network (psv_conv.hip).
 // NHWC bfloat16 activations with images stacked along the rows.
 enum ConvMode { kConv = 0, kScatter = 1, kGather2x2 = 2 };

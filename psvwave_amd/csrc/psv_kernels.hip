@@ -20,6 +20,7 @@
 // independent cross-check of the fused kernel and as a tuning alternative.
 #include "psv_kernels.h"
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_ext.h>
 
 #include <cmath>
@@ -2073,6 +2074,38 @@ batch_lean(BatchLaunch b) {
   lean_tile<TZ, CORR, HALO_DMA>(a, mt.y, b.it);
 }
 
+// Persistent batched forward: one cooperative launch advances steps
+// [k0, k1) of one shot for every member.  Each block walks its share of the
+// (member, tile) pairs -- the blocks of one XCD a contiguous range, as
+// block_tile maps them -- and the grid meets at a barrier between steps,
+// since a step reads what every block wrote in the previous one.  The grid
+// holds no more blocks than fit on the device at once (the cooperative
+// launch checks it), so every block reaches every barrier.
+// One tile of one member at step k.
+template <int TZ>
+__device__ __forceinline__ void persistent_tile(const BatchLaunch &b, int g, int k) {
+  const int2 mt = b.tiles[g];
+  const StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + (k & 1)];
+  lean_tile<TZ, false>(a, mt.y, k);
+}
+
+template <int TZ, int WPE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
+batch_lean_persistent(BatchLaunch b, int k0, int k1) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const int per_xcd = gridDim.x / 8;  // the grid is a multiple of 8 blocks
+  const int chunk = (b.ntiles + 7) / 8;
+  const int t0 = (blockIdx.x % 8) * chunk;
+  const int t1 = min(b.ntiles, t0 + chunk);
+  for (int k = k0; k < k1; ++k) {
+    for (int g = t0 + (int)blockIdx.x / 8; g < t1; g += per_xcd) {
+      persistent_tile<TZ>(b, g, k);
+      __syncthreads();  // the next tile reuses the LDS stage
+    }
+    grid.sync();
+  }
+}
+
 // Running sums of receiver increments (src/fdModel.cpp:582-595 order): one
 // thread per trace.
 __global__ void receiver_scan_kernel(double *rx, double *rz, int nr, int nt) {
@@ -2484,6 +2517,22 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
   }
 #undef PSV_BATCH_LEAN
   return hipGetLastError();
+}
+
+hipError_t launch_batch_persistent(const BatchLaunch &b, int k0, int k1, hipStream_t s) {
+  auto kernel = batch_lean_persistent<8, 3>;
+  int per_cu = 0, cus = 0, dev = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+  if (e == hipSuccess) e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  int grid = per_cu * cus;
+  grid -= grid % 8;
+  grid = std::min(grid, 8 * ((b.ntiles + 7) / 8));
+  if (grid < 8) return hipErrorInvalidConfiguration;
+  BatchLaunch bl = b;
+  void *args[] = {(void *)&bl, (void *)&k0, (void *)&k1};
+  return hipLaunchCooperativeKernel((const void *)kernel, dim3(grid), dim3(256), args, 0, s);
 }
 
 hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,

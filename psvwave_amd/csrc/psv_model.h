@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <functional>
 #include <map>
 #include <string>
@@ -208,22 +209,35 @@ class Model {
   std::vector<hipEvent_t> ckpt_events_;
   int *scratch_off_ = nullptr; // all-zero CSR offsets
 
-  // asynchronous per-shot calls (see set_async)
+  // asynchronous per-shot calls (see set_async): each call adds a job --
+  // one time loop on one lane, or the replay of deferred kernel terms --
+  // and pump() enqueues the jobs' time steps in chunks, alternating between
+  // the lanes so that neither stream's queue runs dry while the host is
+  // blocked on the other one.  A job waits for its dependencies (jobs added
+  // before it) through their end events.
+  struct LaneJob {
+    int kind;    // kJobForward, kJobAdjoint, kJobDeferred, kJobReplay
+    int shot;
+    bool store;
+    int dep[2];  // ids of jobs that must end first, -1 for none
+    int lane, k, parity;
+    bool started, done;
+    StepArgs a;
+  };
   bool async_ = true;
-  bool lane_busy_[2] = {false, false};  // work enqueued since the last settle
-  std::vector<int> fwd_pending_;        // per shot: 0, or 1 + lane of its forward
-  std::vector<hipEvent_t> fwd_events_;  // end of each shot's pending forward
+  std::vector<LaneJob> jobs_;           // this batch of calls, by id
+  std::deque<int> lane_queue_[2];       // unfinished job ids per lane, in order
+  std::vector<hipEvent_t> job_events_;  // end event of each job
+  std::vector<int> fwd_job_;            // per shot: its pending forward job, or -1
   int fwd_calls_ = 0, adj_calls_ = 0;   // since the last settle
-  bool defer_pending_ = false;          // lane Y's adjoint terms not yet added
-  bool replay_enqueued_ = false;        // ev_replayed_ marks the last replay
+  int deferred_job_ = -1;               // deferred adjoint not yet replayed
+  int last_replay_job_ = -1;            // its deferred-term buffer is free after it
   bool no_defer_room_ = false;          // the device had no room for defer_
-  hipEvent_t ev_defer_done_ = nullptr, ev_replayed_ = nullptr, ev_join_ = nullptr;
+  hipEvent_t ev_join_ = nullptr;
   bool async_path(bool verbose, bool dump) const;
   void ensure_lane();
-  void wait_forward(int i_shot, int lane);
-  void enqueue_replay(hipStream_t s);
-  void enqueue_loop_on(StepArgs &a, int i_shot, bool forward, bool store, hipStream_t s,
-                       double *(*st)[5], double *defer);
+  int push_job(int lane, int kind, int shot, bool store, int dep0, int dep1);
+  void pump(bool all);
 };
 
 }  // namespace psv

@@ -94,9 +94,8 @@ Model::~Model() {
   for (auto e : events_) (void)hipEventDestroy(e);
   for (auto e : lane_events_) (void)hipEventDestroy(e);
   for (auto e : ckpt_events_) (void)hipEventDestroy(e);
-  for (auto e : fwd_events_) (void)hipEventDestroy(e);
-  for (auto e : {ev_defer_done_, ev_replayed_, ev_join_})
-    if (e) (void)hipEventDestroy(e);
+  for (auto e : job_events_) (void)hipEventDestroy(e);
+  if (ev_join_) (void)hipEventDestroy(ev_join_);
   if (lane_stream_) (void)hipStreamDestroy(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
@@ -611,27 +610,9 @@ void Model::enqueue_loop(StepArgs &a, int i_shot, bool forward, bool store,
               "receiver scan");
 }
 
-// One whole time loop on stream s with state buffers st (the prologue of
-// run_model_two_lanes' jobs): the adjoint source of an adjoint's shot is in
-// place already; with `defer` the adjoint's kernel terms go to that zeroed
-// per-slot buffer instead of the kernels.
-void Model::enqueue_loop_on(StepArgs &a, int i_shot, bool forward, bool store,
-                            hipStream_t s, double *(*st)[5], double *defer) {
-  for (int k = 0; k < 2; ++k)
-    for (int f = 0; f < 5; ++f)
-      hip_check(launch_fill(st[k][f], npad_, 0.0, s), "zero state");
-  if (defer) hip_check(launch_fill(defer_, defer_n_, 0.0, s), "zero deferred terms");
-  a.rec_increments = 0;
-  a.rec_off = adj_off_;
-  a.rec_entries = adj_entries_;
-  SlotFn slot = [this, i_shot](int it, double **dst) {
-    return stored_slot(i_shot, it, dst);
-  };
-  enqueue_range(a, st, 0, forward, 0, cfg_.nt, store || !forward, slot, nullptr, false, s,
-                defer);
-}
-
 // ---- asynchronous per-shot calls ----
+
+enum { kJobForward, kJobAdjoint, kJobDeferred, kJobReplay };
 
 void Model::set_async(bool on) {
   settle();
@@ -645,67 +626,131 @@ bool Model::async_path(bool verbose, bool dump) const {
          ckpt_every_ == 1 && !step_is_two_step(variant_) && !step_is_inplace(variant_);
 }
 
-// Lane Y: its stream, state buffers and the per-shot completion events.
-// Allocation zero-fills on the model stream, so the first use waits for it.
+// Lane Y: its stream and state buffers.  Allocation zero-fills on the model
+// stream, so the first use waits for it.
 void Model::ensure_lane() {
-  bool fresh = false;
   if (!lane_stream_)
     hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
               "hipStreamCreate");
+  if (!ev_join_) hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming),
+                           "hipEventCreate");
+  if ((int)fwd_job_.size() < cfg_.n_shots) fwd_job_.assign(cfg_.n_shots, -1);
   if (!lane_state_[0][0]) {
     for (int p = 0; p < 2; ++p)
       for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
-    fresh = true;
+    sync();
   }
-  while ((int)fwd_events_.size() < cfg_.n_shots) {
+}
+
+int Model::push_job(int lane, int kind, int shot, bool store, int dep0, int dep1) {
+  LaneJob j;
+  j.kind = kind;
+  j.shot = shot;
+  j.store = store;
+  j.dep[0] = dep0;
+  j.dep[1] = dep1;
+  j.lane = lane;
+  j.k = j.parity = 0;
+  j.started = j.done = false;
+  const int id = (int)jobs_.size();
+  jobs_.push_back(j);
+  while (job_events_.size() < jobs_.size()) {
     hipEvent_t e;
     hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    fwd_events_.push_back(e);
+    job_events_.push_back(e);
   }
-  for (hipEvent_t *e : {&ev_defer_done_, &ev_replayed_, &ev_join_})
-    if (!*e) hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
-  if ((int)fwd_pending_.size() < cfg_.n_shots) fwd_pending_.assign(cfg_.n_shots, 0);
-  if (fresh) sync();
+  lane_queue_[lane].push_back(id);
+  return id;
 }
 
-// Lane `lane` (0 = X, the model stream; 1 = Y) waits for shot i_shot's
-// pending forward when that ran on the other lane.
-void Model::wait_forward(int i_shot, int lane) {
-  const int f = fwd_pending_[i_shot];
-  if (f && f - 1 != lane)
-    hip_check(hipStreamWaitEvent(lane ? lane_stream_ : stream_, fwd_events_[i_shot], 0),
-              "hipStreamWaitEvent");
+// Enqueues the jobs' time steps, 64 at a time per lane in turn: one round
+// (all = false) or until every job is enqueued.  The oldest unfinished job
+// never waits on a younger one, so every round makes progress.
+void Model::pump(bool all) {
+  constexpr int kChunk = 64;
+  const int nt = cfg_.nt;
+  for (;;) {
+    bool progress = false;
+    for (int l = 0; l < 2; ++l) {
+      if (lane_queue_[l].empty()) continue;
+      LaneJob &jb = jobs_[lane_queue_[l].front()];
+      const int id = lane_queue_[l].front();
+      if ((jb.dep[0] >= 0 && !jobs_[jb.dep[0]].done) ||
+          (jb.dep[1] >= 0 && !jobs_[jb.dep[1]].done))
+        continue;
+      hipStream_t s = l ? lane_stream_ : stream_;
+      double *(*st)[5] = l ? lane_state_ : state_;
+      if (!jb.started) {
+        for (int d : jb.dep)
+          if (d >= 0 && jobs_[d].lane != l)
+            hip_check(hipStreamWaitEvent(s, job_events_[d], 0), "hipStreamWaitEvent");
+        jb.started = true;
+        if (jb.kind == kJobReplay) {
+          // the deferred adjoint's kernel terms, in its own order
+          hip_check(launch_replay_correlation(
+                        rho_l_, lam_k_, mu_k_, defer_, d_.snapshots, ngrid_, d_.nx,
+                        cfg_.np_boundary + cfg_.nx_inner_boundary,
+                        cfg_.np_boundary + cfg_.nx_inner - cfg_.nx_inner_boundary,
+                        cfg_.np_boundary + cfg_.nz_inner_boundary,
+                        cfg_.np_boundary + cfg_.nz_inner - cfg_.nz_inner_boundary, s),
+                    "replay correlation");
+          jb.k = nt;
+        } else {
+          jb.a = jb.kind == kJobForward ? forward_args(jb.shot) : adjoint_args(jb.shot);
+          jb.a.rec_increments = 0;
+          jb.a.rec_off = adj_off_;
+          jb.a.rec_entries = adj_entries_;
+          for (int p = 0; p < 2; ++p)
+            for (int f = 0; f < 5; ++f)
+              hip_check(launch_fill(st[p][f], npad_, 0.0, s), "zero state");
+          if (jb.kind == kJobDeferred)
+            hip_check(launch_fill(defer_, defer_n_, 0.0, s), "zero deferred terms");
+        }
+      }
+      if (jb.k < nt) {
+        const int shot = jb.shot;
+        SlotFn slot = [this, shot](int it, double **dst) {
+          return stored_slot(shot, it, dst);
+        };
+        const bool fwd = jb.kind == kJobForward;
+        const int k1 = std::min(jb.k + kChunk, nt);
+        jb.parity = enqueue_range(jb.a, st, jb.parity, fwd, jb.k, k1, jb.store || !fwd, slot,
+                                  nullptr, false, s,
+                                  jb.kind == kJobDeferred ? defer_ : nullptr);
+        jb.k = k1;
+      }
+      progress = true;
+      if (jb.k == nt) {
+        hip_check(hipEventRecord(job_events_[id], s), "hipEventRecord");
+        jb.done = true;
+        lane_queue_[l].pop_front();
+      }
+    }
+    if (!all || !progress) break;
+  }
 }
 
-// Adds lane Y's deferred adjoint terms to the kernels on stream s (after
-// that adjoint has finished), in the adjoint's own order.
-void Model::enqueue_replay(hipStream_t s) {
-  hip_check(hipStreamWaitEvent(s, ev_defer_done_, 0), "hipStreamWaitEvent");
-  hip_check(launch_replay_correlation(
-                rho_l_, lam_k_, mu_k_, defer_, d_.snapshots, ngrid_, d_.nx,
-                cfg_.np_boundary + cfg_.nx_inner_boundary,
-                cfg_.np_boundary + cfg_.nx_inner - cfg_.nx_inner_boundary,
-                cfg_.np_boundary + cfg_.nz_inner_boundary,
-                cfg_.np_boundary + cfg_.nz_inner - cfg_.nz_inner_boundary, s),
-            "replay correlation");
-  hip_check(hipEventRecord(ev_replayed_, s), "hipEventRecord");
-  defer_pending_ = false;
-  replay_enqueued_ = true;
-}
-
-// Waits for every enqueued per-shot loop and adds pending deferred terms.
-// Errors of the enqueued work surface here.
+// Enqueues every pending job, adds pending deferred terms and waits for the
+// device.  Errors of the enqueued work surface here.
 void Model::settle() {
-  if (!lane_busy_[0] && !lane_busy_[1] && !defer_pending_) return;
-  if (defer_pending_) enqueue_replay(stream_);
-  if (lane_busy_[1]) {
+  if (jobs_.empty()) return;
+  if (deferred_job_ >= 0) {
+    push_job(0, kJobReplay, -1, false, deferred_job_, -1);
+    deferred_job_ = -1;
+  }
+  pump(true);
+  bool y = false;
+  for (const LaneJob &j : jobs_) y = y || j.lane == 1;
+  jobs_.clear();
+  lane_queue_[0].clear();
+  lane_queue_[1].clear();
+  std::fill(fwd_job_.begin(), fwd_job_.end(), -1);
+  fwd_calls_ = adj_calls_ = 0;
+  last_replay_job_ = -1;
+  if (y) {
     hip_check(hipEventRecord(ev_join_, lane_stream_), "hipEventRecord");
     hip_check(hipStreamWaitEvent(stream_, ev_join_, 0), "hipStreamWaitEvent");
   }
-  lane_busy_[0] = lane_busy_[1] = false;
-  std::fill(fwd_pending_.begin(), fwd_pending_.end(), 0);
-  fwd_calls_ = adj_calls_ = 0;
-  replay_enqueued_ = false;
   sync();
 }
 
@@ -974,19 +1019,14 @@ void Model::forward(int i_shot, bool store_fields, bool verbose,
     // forward of the same shot writes the same traces: wait for them; a new
     // snapshot store is zero-filled on the model stream before any lane
     // writes to it
-    if (adj_calls_ > 0 || fwd_pending_[i_shot] || (store_fields && !snap_[0])) settle();
+    if (adj_calls_ > 0 || fwd_job_[i_shot] >= 0 || (store_fields && !snap_[0])) settle();
     if (store_fields && !snap_[0]) {
       ensure_snapshots();
       sync();
     }
-    const int lane = fwd_calls_ & 1;
-    hipStream_t s = lane ? lane_stream_ : stream_;
-    StepArgs a = forward_args(i_shot);
-    enqueue_loop_on(a, i_shot, true, store_fields, s, lane ? lane_state_ : state_, nullptr);
-    hip_check(hipEventRecord(fwd_events_[i_shot], s), "hipEventRecord");
-    fwd_pending_[i_shot] = 1 + lane;
-    lane_busy_[lane] = true;
+    fwd_job_[i_shot] = push_job(fwd_calls_ & 1, kJobForward, i_shot, store_fields, -1, -1);
     ++fwd_calls_;
+    pump(false);
     return;
   }
   settle();
@@ -1020,23 +1060,20 @@ void Model::adjoint(int i_shot, bool verbose) {
       sync();
     }
     const int lane = defer_ ? (adj_calls_ & 1) : 0;
-    StepArgs a = adjoint_args(i_shot);
+    const int fwd = fwd_job_[i_shot];  // this shot's forward, if pending
     if (lane == 0) {
       // the previous adjoint's deferred terms go into the kernels first
-      if (defer_pending_) enqueue_replay(stream_);
-      wait_forward(i_shot, 0);
-      enqueue_loop_on(a, i_shot, false, false, stream_, state_, nullptr);
+      if (deferred_job_ >= 0) {
+        last_replay_job_ = push_job(0, kJobReplay, -1, false, deferred_job_, -1);
+        deferred_job_ = -1;
+      }
+      push_job(0, kJobAdjoint, i_shot, false, fwd, -1);
     } else {
       // the deferred-term buffer is free once the last replay has read it
-      if (replay_enqueued_)
-        hip_check(hipStreamWaitEvent(lane_stream_, ev_replayed_, 0), "hipStreamWaitEvent");
-      wait_forward(i_shot, 1);
-      enqueue_loop_on(a, i_shot, false, false, lane_stream_, lane_state_, defer_);
-      hip_check(hipEventRecord(ev_defer_done_, lane_stream_), "hipEventRecord");
-      defer_pending_ = true;
+      deferred_job_ = push_job(1, kJobDeferred, i_shot, false, fwd, last_replay_job_);
     }
-    lane_busy_[lane] = true;
     ++adj_calls_;
+    pump(false);
     return;
   }
   settle();

@@ -37,13 +37,15 @@ def _fields(nx, nz, seed):
                                            (9, False), (8, True), (10, False),
                                            (11, False), (12, False), (13, False),
                                            (18, False), (19, False), (20, False)])
-def test_batch_members_equal_single_models(oracle_lib, variant, graph):
+def test_batch_members_equal_single_models(oracle_lib, variant, graph, persistent=False):
     import psvwave_amd
     cfgs = _members()
     batch = psvwave_amd.Batch(cfgs)
     batch.set_variant(variant)
     if graph:
         batch.set_graph_mode(True)
+    if persistent:
+        batch.set_persistent(True)
     singles = []
     for m, cfg in enumerate(cfgs):
         info = batch.infos[m]
@@ -72,6 +74,17 @@ def test_batch_members_equal_single_models(oracle_lib, variant, graph):
         o.forward_simulate(s, store_fields=False)
     for a, b in zip(batch.get_synthetic_data(3), o.get_synthetic_data()):
         assert np.array_equal(a, b)
+
+
+def test_persistent_batch_loop_equals_single_models(oracle_lib):
+    """One cooperative launch per shot's loop, a grid barrier between
+    steps: the same results as per-step launches and as single models."""
+    test_batch_members_equal_single_models(oracle_lib, 9, False, persistent=True)
+    import psvwave_amd
+    batch = psvwave_amd.Batch(_members()[:2])
+    batch.set_variant(10)
+    with pytest.raises(ValueError, match="variant 9"):
+        batch.set_persistent(True)
 
 
 def test_batch_autotune_and_validation():
