@@ -104,8 +104,8 @@ def _args():
                         "unet: the preconditioner network's training step and "
                         "inference on the library's MFMA kernels vs torch")
     p.add_argument("--c4-directions", type=int, default=20)
-    p.add_argument("--c5-pairs", type=int, default=10240)
-    p.add_argument("--c5-epochs", type=int, default=12)
+    p.add_argument("--c5-pairs", type=int, default=8192)
+    p.add_argument("--c5-epochs", type=int, default=16)
     p.add_argument("--c5-iterations", type=int, default=30)
     p.add_argument("--c5-along-path", type=float, default=0.5,
                    help="fraction of the training pairs taken part of the way "
@@ -453,7 +453,9 @@ def learned_fwi(a):
                 "learned_seconds_to_target": round(t_learn, 2) if t_learn is not None else None,
                 "learned_iterations": len(h_learn) - 1,
                 "fallbacks": int(sum(1 for x in h_learn if x[2])),
-                "initial_misfit": h_plain[0][1], "target_misfit": target})
+                "initial_misfit": h_plain[0][1], "target_misfit": target,
+                "plain_history": [[round(t, 3), f] for t, f, _ in h_plain],
+                "learned_history": [[round(t, 3), f, bool(fb)] for t, f, fb in h_learn]})
             _progress(f"model {h} seed {seed}: ratio {runs[-1]['ratio']}")
     ratios = [r["ratio"] if r["ratio"] is not None else float("inf") for r in runs]
     med = float(np.median(ratios))
@@ -685,7 +687,8 @@ def main():
                    14: "step_lean2<8>", 15: "step_lean_pipe<8,2>",
                    16: "step_lean_pipe<8,3>", 17: "step_lean_pipe<8,4>",
                    18: "step_lean<4>", 19: "step_lean<8,6 waves>",
-                   20: "step_lean<12,5 waves>",
+                   20: "step_lean<12,5 waves>", 21: "step_lean<8,halo DMA>",
+                   22: "step_lean<8,separate stages>", 23: "step_lean<4,separate stages>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -207,7 +207,9 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * 2^40]), 14 the lean two-step kernel, 15-17 lean kernels whose blocks
  * advance 2, 3 or 4 tiles of a column strip in turn (the next tile's loads
  * in flight meanwhile), 18 the lean kernel on 64 x 4 tiles, 19 / 20 the
- * 64 x 8 / 64 x 12 lean kernels held to 6 / 5 waves per SIMD.
+ * 64 x 8 / 64 x 12 lean kernels held to 6 / 5 waves per SIMD, 21 the 64 x 8
+ * lean kernel with halo columns in the row transfers, 22 / 23 the 64 x 8 /
+ * 64 x 4 lean kernels with separate velocity and stress LDS stages.
  * All variants give bitwise identical results.  xcd_remap: 1 keeps
  * neighbouring tiles on one XCD.
  * timing_stride k > 0 records the kernel start and end of every k-th step
@@ -288,8 +290,8 @@ int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
  * with kernel correlation -- one launch per time step over every member --
  * then misfits (psv_batch_misfits) and velocity kernels
  * (psv_batch_get_kernels, (nx, nz) each).  Needs one snapshot_interval for
- * all members, dividing nt, and a one-step variant; each member keeps one
- * shot's snapshots.  Results equal each member's run_model bit for bit. */
+ * all members, dividing nt, and step variant 0, 1, 9, 10 or 18-23; each
+ * member keeps one shot's snapshots.  Results equal each member's run_model bit for bit. */
 int psv_batch_gradients(psv_batch *b);
 int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
                           double *rho_k);
@@ -298,7 +300,7 @@ int psv_batch_set_graph_mode(psv_batch *b, int on);
  * resident blocks meet at a grid barrier between steps, instead of one
  * launch per step (step variant 9 only; the same results). */
 int psv_batch_set_persistent(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20 (ms_per_step[13],
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22, 23 (ms_per_step[15],
  * -1 for a variant the batch cannot use) and keeps the fastest (*best).
  * Overwrites shot 0's synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,

@@ -14,7 +14,7 @@ from . import _lib
 from .model import _f64
 
 # step variants Batch.autotune times, in the library's order
-AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20)
+AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23)
 
 
 class Batch:
@@ -117,7 +117,7 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 
     def autotune(self, steps=20):
-        """Time variants 0, 1, 6, 7, 8-13, 18-20 over `steps` steps and keep the fastest.
+        """Time variants 0, 1, 6, 7, 8-13, 18-20, 22, 23 over `steps` steps and keep the fastest.
         Returns (best variant, {variant: ms per step})."""
         best = C.c_int()
         ms = np.zeros(len(AUTOTUNE_VARIANTS))

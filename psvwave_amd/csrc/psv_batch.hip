@@ -210,8 +210,8 @@ void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
 }
 
 void Batch::gradients() {
-  if (step_is_two_step(variant_))
-    throw ValueError("batched gradients need a one-step variant (0, 1, 8-13, 18-20)");
+  if (!batch_gradient_variant_ok(variant_))
+    throw ValueError("batched gradients need step variant 0, 1, 9, 10, 18-23");
   si_ = models_[0]->config().snapshot_interval;
   for (auto &m : models_)
     if (m->config().snapshot_interval != si_)
@@ -268,7 +268,7 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
   const int candidates[kBatchCandidates] = {
       kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16, kLeanTz16,
       kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32,
-      kLeanTz4,  kLeanTz8W6, kLeanTz12W5};
+      kLeanTz4,  kLeanTz8W6, kLeanTz12W5, kLeanTz8Sep, kLeanTz4Sep};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;

@@ -53,7 +53,9 @@ struct StepArgs {
   // FastDiv split reciprocals: zh = RD(1/d), zl = RN(1/d - zh) >= 0
   double zhx, zlx, zhz, zlz;
   // taper factors by column and by row: taper(x, z) = min(taper_x[x],
-  // taper_z[z]) (the table is nondecreasing in the nesting count)
+  // taper_z[z]) (the table is nondecreasing in the nesting count); valid
+  // (margins hold 1) for x in [-kHaloX, pitch - kHaloX) and z in
+  // [-kHaloZ, rows - kHaloZ)
   const double *taper_x, *taper_z;
   int it, nt, adjoint;
   // snapshot slot of this step (compact [iz][ix]); written forward, read
@@ -182,7 +184,9 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 // lean two-step kernel (64 x 8 tiles), 15-17 the pipelined lean kernels,
 // 18 the lean kernel on 64 x 4 tiles (7 waves per SIMD), 19 and 20 the
 // 64 x 8 and 64 x 12 lean kernels held to 6 and 5 waves per SIMD, 21 the
-// 64 x 8 lean kernel with the velocity halo columns in the row transfers.
+// 64 x 8 lean kernel with the velocity halo columns in the row transfers,
+// 22 / 23 the 64 x 8 / 64 x 4 lean kernels with separate velocity and
+// stress LDS stages.
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -206,18 +210,22 @@ enum StepVariant {
   kLeanTz8W6 = 19,
   kLeanTz12W5 = 20,
   kLeanTz8Halo = 21,
-  kNumVariants = 22
+  kLeanTz8Sep = 22,
+  kLeanTz4Sep = 23,
+  kNumVariants = 24
 };
 constexpr int kMaxTileHeight = 32;
 
 // Lean one-step kernels (variants 8-13): lane = column, FastDiv, reduced
 // stress ring.  They need FastDiv's divisor range and a taper table that is
 // nondecreasing in the nesting count; the host checks both.
-bool step_is_lean(int variant);       // lean one-step kernels (8-13, 18-21)
+bool step_is_lean(int variant);       // lean one-step kernels (8-13, 18-23)
 bool step_uses_fastdiv(int variant);  // lean one- and two-step kernels
 // Pipelined lean kernels (15-17): 64 x 8 tiles, a block advances 2, 3 or 4
 // tiles of one column strip in turn, fetching the next while one computes.
 bool step_is_lean_pipe(int variant);
+// Step variants batched gradients can use (the correlating batch kernels).
+bool batch_gradient_variant_ok(int variant);
 bool fast_division_ok(double d);
 void fast_division_constants(double d, double *zh, double *zl);
 

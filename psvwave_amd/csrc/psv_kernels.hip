@@ -530,18 +530,28 @@ __device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
 
 // HALO_DMA: the halo columns of the velocity rows come in the rows' own
 // transfers (70-column rows) instead of through registers.
-template <int TZ, bool CORR, bool HALO_DMA = false>
+// SEP: the stresses get an LDS stage of their own beside the velocities, so
+// they are written as soon as computed (no barrier between the two uses of
+// one stage) and the velocity pass reads the start-of-step velocities back
+// from LDS instead of keeping them in registers.
+template <int TZ, bool CORR, bool HALO_DMA = false, bool SEP = false>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, const int it) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
   // One LDS buffer: velocities while the stresses are computed, then the
-  // stresses for the velocity update (a barrier separates the two uses).
+  // stresses for the velocity update (a barrier separates the two uses);
+  // two stages side by side with SEP.
   constexpr int NV = 2 * L::VR * L::VW;
   constexpr int NS = TZ * L::XXW + L::ZZR * 64 + L::XZR * L::XZW;
-  __shared__ double lds[NV > NS ? NV : NS];
+  constexpr int NI = SEP ? 16 : kInjStage;  // staged injection entries
+  __shared__ double lds[SEP ? NV + NS : (NV > NS ? NV : NS)];
   double *const s_vx = lds, *const s_vz = lds + L::VR * L::VW;
-  double *const s_xx = lds, *const s_zz = lds + TZ * L::XXW;
+  double *const s_xx = SEP ? lds + NV : lds, *const s_zz = s_xx + TZ * L::XXW;
   double *const s_xz = s_zz + L::ZZR * 64;
+  // the tile's taper factors: columns x0-2 .. x0+65, then rows z0-2 ..
+  // z0+TZ+1, staged by DMA like the velocity rows
+  constexpr int TXN = 68, TZN = TZ + 4;
+  __shared__ double s_tap[TXN + TZN];
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -556,6 +566,30 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
 
   // ---- phase A: every global load before the first wait ----
+  {
+    const unsigned lane4 = (threadIdx.x & 63) * 4u;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (wv == 0) {  // 136 dwords of taper_x
+      const char *g = (const char *)(a.taper_x + x0 - 2) + lane4;
+      __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)s_tap, 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(s_tap + 32), 4,
+                                       0, 0);
+      if (lane4 < 32)
+        __builtin_amdgcn_global_load_lds((const void *)(g + 512), (lds_void *)(s_tap + 64), 4,
+                                         0, 0);
+    } else if (wv == 1) {  // 2 TZN dwords of taper_z
+      const char *g = (const char *)(a.taper_z + z0 - 2) + lane4;
+      if (lane4 < 8u * TZN)
+        __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)(s_tap + TXN), 4, 0, 0);
+      if (2 * TZN > 64 && lane4 + 256 < 8u * TZN)
+        __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(s_tap + TXN + 32),
+                                         4, 0, 0);
+    }
+  }
+  // This tile's injection list bounds first: they are waited for before the
+  // staging below, and counters retire in issue order.
+  const int tile_u = __builtin_amdgcn_readfirstlane(tile);
+  const int e0 = a.tile_off[tile_u], ne = a.tile_off[tile_u + 1] - e0;
   // Velocity rows go straight from memory into LDS (no registers): wave w
   // stages rows r = w + 4m, m = -1 ..; its own rows are r = w + 4j,
   // j = 0 .. R-1.
@@ -652,15 +686,16 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   const int x = x0 + lane;
   const bool xin = x < a.nx;
   const bool xint = x >= 2 && x < a.nx - 2;
-  const double tx = a.taper_x[xin ? x : 0];
+  // taper factors from the LDS segments: column c, row r of the tile
+  auto tap_x = [&](int c) { return s_tap[c + 2]; };
+  auto tap_z = [&](int r) { return s_tap[TXN + r + 2]; };
   if (!a.adjoint && tile == 0) record_receivers(a, it);
   // This tile's injection entries and their source samples at this step,
   // fetched by all threads at once and staged in LDS; phase C applies them
   // in list order.
-  __shared__ double s_ival[kInjStage], s_icoef[kInjStage];
-  __shared__ int s_iloc[kInjStage];
-  const int e0 = a.tile_off[tile], ne = a.tile_off[tile + 1] - e0;
-  for (int t = threadIdx.x; t < ne && t < kInjStage; t += 256) {
+  __shared__ double s_ival[NI], s_icoef[NI];
+  __shared__ int s_iloc[NI];
+  for (int t = threadIdx.x; t < ne && t < NI; t += 256) {
     const InjEntry E = a.entries[e0 + t];
     s_iloc[t] = E.local | (E.comp << 16);
     s_icoef[t] = E.coef;
@@ -675,6 +710,10 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     }
   __builtin_amdgcn_s_waitcnt(0);  // the LDS DMA of this wave has landed
   __syncthreads();
+  const double tx = tap_x(lane);
+  double trow[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) trow[j] = tap_z(w + 4 * j);
   // start-of-step velocities of the own points
   double vx0[R], vz0[R];
 #pragma unroll
@@ -689,12 +728,6 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   const double dts = a.dts;
   const IeeeDiv ix{a.dx}, iz{a.dz};
   auto zint = [&](int z) { return z >= 2 && z < a.nz - 2; };
-  auto taper_row = [&](int z) {
-    return a.taper_z[z < 0 ? 0 : (z >= a.nz ? a.nz - 1 : z)];
-  };
-  double trow[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) trow[j] = taper_row(z0 + w + 4 * j);
   if (CORR) {
 #pragma unroll
     for (int j = 0; j < R; ++j)
@@ -733,13 +766,17 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
                                               ldb(mup, o), fmin(tx, trow[j]), dts, ix,
                                               iz, oxx[j], ozz[j], oxz[j]);
       }
+      // the reloads have landed before the paths join, so that later code
+      // never waits on them (counters would otherwise make it wait for every
+      // store issued since)
+      __builtin_amdgcn_s_waitcnt(0);
     }
   }
   // ring row (its new values replace the loaded ones)
   {
     const int z = z0 + rr;
     if (xint && zint(z)) {
-      const double t = fmin(tx, taper_row(z));
+      const double t = fmin(tx, tap_z(rr));
       const int v = L::V(rr, lane);
       DivFlags gf;
       const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
@@ -749,6 +786,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
         gzz = ldb(tzzi, o), gxz = ldb(txzi, o);
         ring_row_stress<L::VW>(w, s_vx, s_vz, v, ldb(lmp, o), ldb(mup, o), t, dts, ix, iz,
                                gzz, gxz);
+        __builtin_amdgcn_s_waitcnt(0);
       }
     }
   }
@@ -761,7 +799,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
       const int c = q < 2 ? q - 2 : 62 + q;
       const int xc = x0 + c, z = z0 + r;
       if (xc >= 2 && xc < a.nx - 2 && zint(z)) {
-        const double t = fmin(a.taper_x[xc], taper_row(z));
+        const double t = fmin(tap_x(c), tap_z(r));
         DivFlags cf;
         const FastDiv cx{a.dx, a.rdx, a.zhx, a.zlx, &cf}, cz{a.dz, a.rdz, a.zhz, a.zlz, &cf};
         double hxx = cxx[p], hxz = cxz[p], hzz = 0.0;
@@ -772,6 +810,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
           hxx = ldb(txxi, o), hxz = ldb(txzi, o);
           lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), ldb(lmp, o), ldb(mup, o),
                                         t, dts, ix, iz, hxx, hzz, hxz);
+          __builtin_amdgcn_s_waitcnt(0);
         }
         cxx[p] = hxx, cxz[p] = hxz;
       }
@@ -790,7 +829,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
       stb(txzo, o, oxz[j]);
     }
   }
-  __syncthreads();  // every wave is done reading the velocity stage
+  if (!SEP) __syncthreads();  // every wave is done reading the velocity stage
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int r = w + 4 * j;
@@ -813,6 +852,13 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   __syncthreads();
 
   // ---- phase C: velocities ----
+  if (SEP) {  // the velocity stage still holds the start-of-step values
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      vx0[j] = s_vx[L::V(w + 4 * j, lane)];
+      vz0[j] = s_vz[L::V(w + 4 * j, lane)];
+    }
+  }
   double vx[R], vz[R];
   {
     DivFlags vf;
@@ -843,7 +889,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   for (int e = 0; e < ne; ++e) {
     int lc;
     double coef, val;
-    if (e < kInjStage) {
+    if (e < NI) {
       lc = s_iloc[e], coef = s_icoef[e], val = s_ival[e];
     } else {
       const InjEntry E = a.entries[e0 + e];
@@ -878,12 +924,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
 
 // CORR: instantiation for adjoint snapshot steps (kernel correlation; one
 // wave per SIMD fewer, for the correlation operands in flight).
-template <int TZ, int WPE, bool CORR, bool HALO_DMA = false>
+template <int TZ, int WPE, bool CORR, bool HALO_DMA = false, bool SEP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ, CORR, HALO_DMA>(a, tile, a.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP>(a, tile, a.it);
 }
 
 // ---------------------------------------------------------------------------
@@ -2063,7 +2109,7 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
   two_step_tile<TZ, NT>(a, mt.y);
 }
 
-template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false>
+template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false, bool SEP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
@@ -2071,7 +2117,7 @@ batch_lean(BatchLaunch b) {
   const int2 mt = b.tiles[g];
   StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
   if (b.snapshots) batch_snapshot_slot(a, b);
-  lean_tile<TZ, CORR, HALO_DMA>(a, mt.y, b.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP>(a, mt.y, b.it);
 }
 
 // Persistent batched forward: one cooperative launch advances steps
@@ -2420,6 +2466,7 @@ int step_tile_height(int variant) {
     case kLeanTz32:
       return 32;
     case kLeanTz4:
+    case kLeanTz4Sep:
       return 4;
     case kLeanTz12W5:
       return 12;
@@ -2429,7 +2476,7 @@ int step_tile_height(int variant) {
 }
 bool step_is_lean(int variant) {
   return (variant >= kLeanTz16 && variant <= kLeanTz32) ||
-         (variant >= kLeanTz4 && variant <= kLeanTz8Halo);
+         (variant >= kLeanTz4 && variant <= kLeanTz4Sep);
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -2452,6 +2499,12 @@ int step_single_variant(int variant) {
 }
 bool step_uses_fastdiv(int variant) {
   return step_is_lean(variant) || variant == kLeanTwoTz8 || step_is_lean_pipe(variant);
+}
+bool batch_gradient_variant_ok(int variant) {
+  return variant == kFusedTz8 || variant == kFusedTz16 || variant == kLeanTz8 ||
+         variant == kLeanTz12 || variant == kLeanTz4 || variant == kLeanTz8Sep ||
+         variant == kLeanTz4Sep || variant == kLeanTz8W6 || variant == kLeanTz12W5 ||
+         variant == kLeanTz8Halo;
 }
 bool step_is_lean_pipe(int variant) {
   return variant >= kLeanPipe2Tz8 && variant <= kLeanPipe4Tz8;
@@ -2484,14 +2537,19 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
     hipExtLaunchKernelGGL((batch_lean<TZ, WC, true>), g, t, 0, s, e0, e1, 0, b);   \
   else                                                                             \
     hipExtLaunchKernelGGL((batch_lean<TZ, W, false>), g, t, 0, s, e0, e1, 0, b);
+  // tiles of 16 rows and more: forward batches only (batched gradients use
+  // the variants of batch_gradient_variant_ok)
+#define PSV_BATCH_LEAN_FWD(TZ, W)                                                  \
+  if (corr) return hipErrorInvalidValue;                                           \
+  hipExtLaunchKernelGGL((batch_lean<TZ, W, false>), g, t, 0, s, e0, e1, 0, b);
   if (step_is_lean(variant)) {
     switch (variant) {
-      case kLeanTz16: PSV_BATCH_LEAN(16, 3, 2) break;
+      case kLeanTz16: PSV_BATCH_LEAN_FWD(16, 3) break;
       case kLeanTz8: PSV_BATCH_LEAN(8, 5, 4) break;
       case kLeanTz12: PSV_BATCH_LEAN(12, 4, 3) break;
-      case kLeanTz20: PSV_BATCH_LEAN(20, 3, 2) break;
-      case kLeanTz24: PSV_BATCH_LEAN(24, 2, 1) break;
-      case kLeanTz32: PSV_BATCH_LEAN(32, 2, 1) break;
+      case kLeanTz20: PSV_BATCH_LEAN_FWD(20, 3) break;
+      case kLeanTz24: PSV_BATCH_LEAN_FWD(24, 2) break;
+      case kLeanTz32: PSV_BATCH_LEAN_FWD(32, 2) break;
       case kLeanTz4: PSV_BATCH_LEAN(4, 7, 5) break;
       case kLeanTz8W6: PSV_BATCH_LEAN(8, 6, 4) break;
       case kLeanTz12W5: PSV_BATCH_LEAN(12, 5, 3) break;
@@ -2500,6 +2558,22 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
           hipExtLaunchKernelGGL((batch_lean<8, 4, true, true>), g, t, 0, s, e0, e1, 0, b);
         else
           hipExtLaunchKernelGGL((batch_lean<8, 6, false, true>), g, t, 0, s, e0, e1, 0, b);
+        break;
+      case kLeanTz8Sep:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<8, 4, true, false, true>), g, t, 0, s, e0, e1, 0,
+                                b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<8, 5, false, false, true>), g, t, 0, s, e0, e1, 0,
+                                b);
+        break;
+      case kLeanTz4Sep:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<4, 5, true, false, true>), g, t, 0, s, e0, e1, 0,
+                                b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<4, 7, false, false, true>), g, t, 0, s, e0, e1, 0,
+                                b);
         break;
       default:
         return hipErrorInvalidValue;
@@ -2516,6 +2590,7 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
       hipExtLaunchKernelGGL((batch_fused<8, 256>), g, t, 0, s, e0, e1, 0, b);
   }
 #undef PSV_BATCH_LEAN
+#undef PSV_BATCH_LEAN_FWD
   return hipGetLastError();
 }
 
@@ -2634,6 +2709,18 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
         hipExtLaunchKernelGGL((step_lean<8, 4, true, true>), g, b, 0, s, e0, e1, 0, a);
       else
         hipExtLaunchKernelGGL((step_lean<8, 6, false, true>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz8Sep:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 4, true, false, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 5, false, false, true>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz4Sep:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<4, 5, true, false, true>), g, b, 0, s, e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<4, 7, false, false, true>), g, b, 0, s, e0, e1, 0, a);
       break;
     case kLeanPipe2Tz8:
     case kLeanPipe3Tz8:

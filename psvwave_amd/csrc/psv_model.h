@@ -152,6 +152,7 @@ class Model {
   double *vp_k_ = nullptr, *vs_k_ = nullptr, *rho_v_ = nullptr;
   double *stf_ = nullptr, *taper_lut_ = nullptr;
   double *taper_x_ = nullptr, *taper_z_ = nullptr;  // per column / per row
+  double *taper_xp_ = nullptr, *taper_zp_ = nullptr; // their padded storage
   bool lean_ok_ = false;
   bool lean_eligible() const;
   double taper_of_count(int c) const;

@@ -159,13 +159,18 @@ void Model::allocate() {
     auto count1 = [&](int i, int n) {
       return std::min(cfg_.np_boundary, 1 + std::min(i, n - 1 - i));
     };
-    std::vector<double> tx(d_.nx), tz(d_.nz);
-    for (int x = 0; x < d_.nx; ++x) tx[x] = taper_of_count(count1(x, d_.nx));
-    for (int z = 0; z < d_.nz; ++z) tz[z] = taper_of_count(count1(z, d_.nz));
-    taper_x_ = dalloc(tx.size());
-    upload(taper_x_, tx.data(), tx.size());
-    taper_z_ = dalloc(tz.size());
-    upload(taper_z_, tz.data(), tz.size());
+    // padded like the stencil arrays (entry kHaloX + x, kHaloZ + z), so the
+    // lean kernels stage a tile's segment with its halo by plain transfers;
+    // the margins hold 1 and are never used as factors
+    std::vector<double> tx(pitch_, 1.0), tz(rows_, 1.0);
+    for (int x = 0; x < d_.nx; ++x) tx[kHaloX + x] = taper_of_count(count1(x, d_.nx));
+    for (int z = 0; z < d_.nz; ++z) tz[kHaloZ + z] = taper_of_count(count1(z, d_.nz));
+    taper_xp_ = dalloc(tx.size());
+    upload(taper_xp_, tx.data(), tx.size());
+    taper_zp_ = dalloc(tz.size());
+    upload(taper_zp_, tz.data(), tz.size());
+    taper_x_ = taper_xp_ + kHaloX;
+    taper_z_ = taper_zp_ + kHaloZ;
   }
   std::vector<long long> ridx(std::max(cfg_.nr, 1), 0);
   for (int r = 0; r < cfg_.nr; ++r)
@@ -353,6 +358,7 @@ bool Model::lean_eligible() const {
 }
 
 void Model::set_step_variant(int variant, int xcd_remap) {
+  settle();
   if (variant < 0 || variant >= kNumVariants)
     throw ValueError("unknown step variant");
   if (step_uses_fastdiv(variant) && !lean_ok_)
@@ -370,6 +376,7 @@ void Model::clear_graphs() {
 }
 
 void Model::set_graph_mode(bool on) {
+  settle();
   if (!on) clear_graphs();
   graph_mode_ = on;
 }
@@ -1104,6 +1111,7 @@ void Model::adjoint(int i_shot, bool verbose) {
 }
 
 double Model::calculate_l2_misfit() {
+  settle();
   hip_check(launch_misfit(syn_[0], syn_[1], obs_[0], obs_[1], nrec_,
                           0.5 * cfg_.dt, misfit_buf_,
                           misfit_buf_ + 2 * kMisfitBlocks, stream_),
@@ -1113,6 +1121,7 @@ double Model::calculate_l2_misfit() {
 }
 
 void Model::calculate_l2_adjoint_sources() {
+  settle();
   hip_check(launch_adjoint_sources(syn_[0], syn_[1], obs_[0], obs_[1], adj_[0],
                                    adj_[1], nrec_, stream_),
             "adjoint sources");
@@ -1120,6 +1129,7 @@ void Model::calculate_l2_adjoint_sources() {
 }
 
 void Model::reset_kernels() {
+  settle();
   hip_check(launch_fill(lam_k_, ngrid_, 0.0, stream_), "reset");
   hip_check(launch_fill(mu_k_, ngrid_, 0.0, stream_), "reset");
   hip_check(launch_fill(rho_l_, ngrid_, 0.0, stream_), "reset");
@@ -1127,6 +1137,7 @@ void Model::reset_kernels() {
 }
 
 void Model::map_kernels_to_velocity() {
+  settle();
   hip_check(launch_map_kernels(vp_, vs_, rho_, lam_k_, mu_k_, rho_l_, vp_k_,
                                vs_k_, rho_v_, ngrid_, stream_),
             "map kernels");
@@ -1166,6 +1177,48 @@ void Model::run_model(bool verbose, bool simulate_adjoint) {
   if (cfg_.nt % cfg_.snapshot_interval != 0)
     throw ValueError("Snapshot interval and size of accumulator don't match!"
                      " (nt must be a multiple of snapshot_interval)");
+  if (!verbose && ckpt_every_ == 1 && timing_stride_ <= 0 && !graph_mode_ &&
+      cfg_.n_shots > 1 && !step_is_two_step(variant_) && !step_is_inplace(variant_)) {
+    // the per-shot calls on two lanes (forwards two at a time, adjoints in
+    // pairs with deferred kernel terms), bracketed by loop timing events
+    settle();
+    const bool timed = timing_stride_ < 0, was_async = async_;
+    while (events_.size() < 2) {
+      hipEvent_t e;
+      hip_check(hipEventCreate(&e), "hipEventCreate");
+      events_.push_back(e);
+    }
+    const int64_t before = launches_;
+    if (timed) hip_check(hipEventRecord(events_[0], stream_), "hipEventRecord");
+    timing_stride_ = 0;
+    async_ = true;
+    try {
+      for (int s = 0; s < cfg_.n_shots; ++s) forward(s, true, false, false);
+      calculate_l2_misfit();
+      if (simulate_adjoint) {
+        calculate_l2_adjoint_sources();
+        reset_kernels();
+        for (int s = 0; s < cfg_.n_shots; ++s) adjoint(s, false);
+        settle();
+        map_kernels_to_velocity();
+      }
+    } catch (...) {
+      timing_stride_ = timed ? -1 : 0;
+      async_ = was_async;
+      throw;
+    }
+    timing_stride_ = timed ? -1 : 0;
+    async_ = was_async;
+    if (timed) {
+      hip_check(hipEventRecord(events_[1], stream_), "hipEventRecord");
+      sync();
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, events_[0], events_[1]), "hipEventElapsedTime");
+      timed_ms_ += ms;
+      timed_ += launches_ - before;
+    }
+    return;
+  }
   if (!verbose && (ckpt_every_ == 1 || !simulate_adjoint) && timing_stride_ <= 0 &&
       cfg_.n_shots > 1) {
     run_model_two_lanes(simulate_adjoint);
@@ -1429,6 +1482,7 @@ void Model::get_parameter_fields(double *vp, double *vs, double *rho) const {
 
 void Model::set_parameter_fields(const double *vp, const double *vs,
                                  const double *rho) {
+  settle();
   to_device_layout(vp_, vp);
   to_device_layout(vs_, vs);
   to_device_layout(rho_, rho);
@@ -1457,12 +1511,14 @@ void Model::get_traces(int which, double *ux, double *uz) const {
 }
 
 void Model::set_traces(int which, const double *ux, const double *uz) {
+  settle();
   double *const *dst = which == 0 ? syn_ : which == 1 ? obs_ : adj_;
   upload(dst[0], ux, nrec_);
   upload(dst[1], uz, nrec_);
 }
 
 void Model::set_checkpointing(int every) {
+  settle();
   if (every < 1) throw ValueError("checkpoint interval must be >= 1");
   if (every == ckpt_every_) return;
   sync();
@@ -1524,6 +1580,7 @@ void Model::get_model_vector(double *v) const {
 }
 
 void Model::set_model_vector(const double *v) {
+  settle();
   double *s = staging(d_.free_parameters);
   upload(s, v, d_.free_parameters);
   hip_check(launch_model_vector(1, vp_, vs_, rho_, s, d_.nx,
@@ -1617,6 +1674,7 @@ void Model::write_kernels() const {
 // src/fdModel.cpp:1053-1126: stream extraction with the reference's
 // good()-state checks.
 void Model::load_receivers(bool verbose) {
+  settle();
   std::vector<double> ux(nrec_), uz(nrec_);
   for (int s = 0; s < cfg_.n_shots; ++s) {
     std::string nx_ = cfg_.observed_data_folder + "/rtf_ux" + std::to_string(s) + ".txt";

@@ -36,7 +36,8 @@ def _fields(nx, nz, seed):
                                            (7, False), (6, True), (8, False),
                                            (9, False), (8, True), (10, False),
                                            (11, False), (12, False), (13, False),
-                                           (18, False), (19, False), (20, False)])
+                                           (18, False), (19, False), (20, False),
+                                           (21, False), (22, False), (23, False)])
 def test_batch_members_equal_single_models(oracle_lib, variant, graph, persistent=False):
     import psvwave_amd
     cfgs = _members()
@@ -92,7 +93,7 @@ def test_batch_autotune_and_validation():
     cfgs = _members()
     batch = psvwave_amd.Batch(cfgs[:2])
     best, times = batch.autotune(steps=6)
-    tuned = {0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20}
+    tuned = {0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23}
     assert best in tuned
     assert set(times) == tuned
     assert all(t > 0 for t in times.values())
@@ -103,7 +104,7 @@ def test_batch_autotune_and_validation():
         batch.set_variant(2)
 
 
-@pytest.mark.parametrize("variant", [9, 0, 18])
+@pytest.mark.parametrize("variant", [9, 0, 18, 22, 23])
 def test_batch_gradients_equal_single_models(oracle_lib, variant):
     """Batched FWI gradients (forward with snapshots, adjoint with kernel
     correlation, shot by shot over all members) equal each member's own
