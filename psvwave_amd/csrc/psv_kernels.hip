@@ -456,12 +456,22 @@ __device__ __forceinline__ unsigned opaque(unsigned v) {
 }
 
 typedef __attribute__((address_space(3))) void lds_void;
+// LDS pointers for the DMA transfers below.  They are formed from a
+// __shared__ array's own address and offset in the LDS address space, never
+// converted from generic pointers (a generic-to-LDS conversion carries a
+// null test for every transfer).
+typedef __attribute__((address_space(3))) double lds_dbl;
+// The LDS address of a pointer into a __shared__ array: the low 32 bits of
+// its generic address.
+__device__ __forceinline__ lds_dbl *lds_ptr(double *p) {
+  return (lds_dbl *)(unsigned)(unsigned long long)p;
+}
 
 // One 64-double row from memory straight into LDS: two transfers of one
 // dword per lane (bytes 0-255, then 256-511).  lane_off is this lane's byte
 // offset of column `lane`; lds_row (column 0 of the row) is wave-uniform.
 __device__ __forceinline__ void dma_row(const double *base, unsigned lane_off,
-                                        double *lds_row) {
+                                        lds_dbl *lds_row) {
   const unsigned lane = threadIdx.x & 63;
   const char *g = (const char *)base + (lane_off - lane * 4u);
   __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)lds_row, 4, 0, 0);
@@ -473,7 +483,7 @@ __device__ __forceinline__ void dma_row(const double *base, unsigned lane_off,
 // from memory straight into LDS: dwords 0-63, 64-127 and 128-139 (12
 // lanes).  row_base is the global address of column -3, lds_row the LDS
 // address of column -3 (both wave-uniform).
-__device__ __forceinline__ void dma_row70(const double *row_base, double *lds_row) {
+__device__ __forceinline__ void dma_row70(const double *row_base, lds_dbl *lds_row) {
   const unsigned lane = threadIdx.x & 63;
   const char *g = (const char *)row_base + lane * 4u;
   __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)lds_row, 4, 0, 0);
@@ -544,15 +554,23 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   constexpr int NV = 2 * L::VR * L::VW;
   constexpr int NS = TZ * L::XXW + L::ZZR * 64 + L::XZR * L::XZW;
   constexpr int NI = SEP ? 16 : kInjStage;  // staged injection entries
-  __shared__ double lds[SEP ? NV + NS : (NV > NS ? NV : NS)];
+  // the tile's taper factors: columns x0-2 .. x0+65, then rows z0-2 ..
+  // z0+TZ+1, staged by DMA like the velocity rows, after the stages
+  constexpr int TXN = 68, TZN = TZ + 4;
+  constexpr int NST = SEP ? NV + NS : (NV > NS ? NV : NS);
+  __shared__ double lds[NST + TXN + TZN];
   double *const s_vx = lds, *const s_vz = lds + L::VR * L::VW;
   double *const s_xx = SEP ? lds + NV : lds, *const s_zz = s_xx + TZ * L::XXW;
   double *const s_xz = s_zz + L::ZZR * 64;
-  // the tile's taper factors: columns x0-2 .. x0+65, then rows z0-2 ..
-  // z0+TZ+1, staged by DMA like the velocity rows
-  constexpr int TXN = 68, TZN = TZ + 4;
-  __shared__ double s_tap[TXN + TZN];
+  double *const s_tap = lds + NST;
+  lds_dbl *const d_vx = lds_ptr(lds), *const d_vz = d_vx + L::VR * L::VW;
+  lds_dbl *const d_tap = d_vx + NST;
 
+  // This tile's injection list bounds, before any memory access of the
+  // kernel: a uniform load that nothing can have clobbered yet is a scalar
+  // load, which the vector-memory counters do not see.
+  const int tile_u = __builtin_amdgcn_readfirstlane(tile);
+  const int e0 = a.tile_off[tile_u], ne = a.tile_off[tile_u + 1] - e0;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
@@ -571,25 +589,21 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (wv == 0) {  // 136 dwords of taper_x
       const char *g = (const char *)(a.taper_x + x0 - 2) + lane4;
-      __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)s_tap, 4, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(s_tap + 32), 4,
+      __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)d_tap, 4, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(d_tap + 32), 4,
                                        0, 0);
       if (lane4 < 32)
-        __builtin_amdgcn_global_load_lds((const void *)(g + 512), (lds_void *)(s_tap + 64), 4,
+        __builtin_amdgcn_global_load_lds((const void *)(g + 512), (lds_void *)(d_tap + 64), 4,
                                          0, 0);
     } else if (wv == 1) {  // 2 TZN dwords of taper_z
       const char *g = (const char *)(a.taper_z + z0 - 2) + lane4;
       if (lane4 < 8u * TZN)
-        __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)(s_tap + TXN), 4, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void *)g, (lds_void *)(d_tap + TXN), 4, 0, 0);
       if (2 * TZN > 64 && lane4 + 256 < 8u * TZN)
-        __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(s_tap + TXN + 32),
+        __builtin_amdgcn_global_load_lds((const void *)(g + 256), (lds_void *)(d_tap + TXN + 32),
                                          4, 0, 0);
     }
   }
-  // This tile's injection list bounds first: they are waited for before the
-  // staging below, and counters retire in issue order.
-  const int tile_u = __builtin_amdgcn_readfirstlane(tile);
-  const int e0 = a.tile_off[tile_u], ne = a.tile_off[tile_u + 1] - e0;
   // Velocity rows go straight from memory into LDS (no registers): wave w
   // stages rows r = w + 4m, m = -1 ..; its own rows are r = w + 4j,
   // j = 0 .. R-1.
@@ -599,12 +613,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     if (r >= -3 && r <= TZ + 2) {
       if (HALO_DMA) {
         const long long ro = tb + (long long)(r + 3) * a.pitch + 5;  // column -3
-        dma_row70(a.vx_i + ro, s_vx + L::V(r, -3));
-        dma_row70(a.vz_i + ro, s_vz + L::V(r, -3));
+        dma_row70(a.vx_i + ro, d_vx + L::V(r, -3));
+        dma_row70(a.vz_i + ro, d_vz + L::V(r, -3));
       } else {
         const unsigned o = off(r, lane);
-        dma_row(vxi, o, s_vx + L::V(r, 0));
-        dma_row(vzi, o, s_vz + L::V(r, 0));
+        dma_row(vxi, o, d_vx + L::V(r, 0));
+        dma_row(vzi, o, d_vz + L::V(r, 0));
       }
     }
   }
@@ -962,12 +976,12 @@ struct LeanRegs {
 // LDS buffer `buf` (vx, then vz) -- the halo columns come in the same
 // transfers as the row.
 template <int TZ>
-__device__ __forceinline__ void lean_dma_tile(const StepArgs &a, int tile, double *buf) {
+__device__ __forceinline__ void lean_dma_tile(const StepArgs &a, int tile, lds_dbl *buf) {
   using L = Lean<TZ>;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int x0 = (tile % a.ntx) * kTileX, z0 = (tile / a.ntx) * TZ;
   const long long tb = a.origin + (long long)(z0 - 3) * a.pitch + (x0 - 3);
-  double *const s_vx = buf, *const s_vz = buf + L::VR * L::VW;
+  lds_dbl *const s_vx = buf, *const s_vz = buf + L::VR * L::VW;
 #pragma unroll
   for (int k = 0; k < L::KV; ++k) {
     const int r = w + 4 * (k - 1);
@@ -1081,7 +1095,7 @@ __device__ __forceinline__ void lean_pipe_strip(const StepArgs &a, const int str
 
   LeanRegs<TZ, CORR> g;
   const int t_first = tz0 * a.ntx + strip;
-  lean_dma_tile<TZ>(a, t_first, lds);
+  lean_dma_tile<TZ>(a, t_first, lds_ptr(lds));
   lean_fetch_tile<TZ, CORR>(a, t_first, it, g);
   if (!a.adjoint && t_first == 0) record_receivers(a, it);
   // velocities of the previous tile, stored after the next tile's barrier
@@ -1109,7 +1123,7 @@ __device__ __forceinline__ void lean_pipe_strip(const StepArgs &a, const int str
       s_icoef[threadIdx.x] = E.coef;
       s_ival[threadIdx.x] = inj_sample(a, E, it);
     }
-    if (k + 1 < nk) lean_dma_tile<TZ>(a, tile + a.ntx, lds + ((k + 1) & 1) * NB);
+    if (k + 1 < nk) lean_dma_tile<TZ>(a, tile + a.ntx, lds_ptr(lds) + ((k + 1) & 1) * NB);
     if (pend) {
       double *vxo = a.vx_o + ptb, *vzo = a.vz_o + ptb;
 #pragma unroll
@@ -1419,16 +1433,16 @@ __device__ __forceinline__ void lean2_tile(const StepArgs &a, const int tile) {
   for (int k = 0; k < L::KA; ++k) {
     const int r = -8 + w + 4 * k;
     const unsigned o = off(r, lane);
-    dma_row(vxi, o, s_vx + L::A(r, 0));
-    dma_row(vzi, o, s_vz + L::A(r, 0));
+    dma_row(vxi, o, lds_ptr(s_vx) + L::A(r, 0));
+    dma_row(vzi, o, lds_ptr(s_vz) + L::A(r, 0));
   }
 #pragma unroll
   for (int k = 0; k < L::K1; ++k) {
     const int r = -6 + w + 4 * k;
     const unsigned o = off(r, lane);
-    dma_row(txxi, o, s_xx + L::B(r, 0));
-    dma_row(tzzi, o, s_zz + L::B(r, 0));
-    dma_row(txzi, o, s_xz + L::B(r, 0));
+    dma_row(txxi, o, lds_ptr(s_xx) + L::B(r, 0));
+    dma_row(tzzi, o, lds_ptr(s_zz) + L::B(r, 0));
+    dma_row(txzi, o, lds_ptr(s_xz) + L::B(r, 0));
   }
   // side columns through registers
   double avx[2], avz[2];
