@@ -290,7 +290,7 @@ int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
  * with kernel correlation -- one launch per time step over every member --
  * then misfits (psv_batch_misfits) and velocity kernels
  * (psv_batch_get_kernels, (nx, nz) each).  Needs one snapshot_interval for
- * all members, dividing nt, and step variant 0, 1, 9, 10 or 18-23; each
+ * all members, dividing nt, and step variant 0, 1, 8-13 or 18-23; each
  * member keeps one shot's snapshots.  Results equal each member's run_model bit for bit. */
 int psv_batch_gradients(psv_batch *b);
 int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,

@@ -211,7 +211,7 @@ void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
 
 void Batch::gradients() {
   if (!batch_gradient_variant_ok(variant_))
-    throw ValueError("batched gradients need step variant 0, 1, 9, 10, 18-23");
+    throw ValueError("batched gradients need step variant 0, 1, 8-13 or 18-23");
   si_ = models_[0]->config().snapshot_interval;
   for (auto &m : models_)
     if (m->config().snapshot_interval != si_)

@@ -2515,10 +2515,7 @@ bool step_uses_fastdiv(int variant) {
   return step_is_lean(variant) || variant == kLeanTwoTz8 || step_is_lean_pipe(variant);
 }
 bool batch_gradient_variant_ok(int variant) {
-  return variant == kFusedTz8 || variant == kFusedTz16 || variant == kLeanTz8 ||
-         variant == kLeanTz12 || variant == kLeanTz4 || variant == kLeanTz8Sep ||
-         variant == kLeanTz4Sep || variant == kLeanTz8W6 || variant == kLeanTz12W5 ||
-         variant == kLeanTz8Halo;
+  return variant == kFusedTz8 || variant == kFusedTz16 || step_is_lean(variant);
 }
 bool step_is_lean_pipe(int variant) {
   return variant >= kLeanPipe2Tz8 && variant <= kLeanPipe4Tz8;
@@ -2551,19 +2548,14 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
     hipExtLaunchKernelGGL((batch_lean<TZ, WC, true>), g, t, 0, s, e0, e1, 0, b);   \
   else                                                                             \
     hipExtLaunchKernelGGL((batch_lean<TZ, W, false>), g, t, 0, s, e0, e1, 0, b);
-  // tiles of 16 rows and more: forward batches only (batched gradients use
-  // the variants of batch_gradient_variant_ok)
-#define PSV_BATCH_LEAN_FWD(TZ, W)                                                  \
-  if (corr) return hipErrorInvalidValue;                                           \
-  hipExtLaunchKernelGGL((batch_lean<TZ, W, false>), g, t, 0, s, e0, e1, 0, b);
   if (step_is_lean(variant)) {
     switch (variant) {
-      case kLeanTz16: PSV_BATCH_LEAN_FWD(16, 3) break;
+      case kLeanTz16: PSV_BATCH_LEAN(16, 3, 2) break;
       case kLeanTz8: PSV_BATCH_LEAN(8, 5, 4) break;
       case kLeanTz12: PSV_BATCH_LEAN(12, 4, 3) break;
-      case kLeanTz20: PSV_BATCH_LEAN_FWD(20, 3) break;
-      case kLeanTz24: PSV_BATCH_LEAN_FWD(24, 2) break;
-      case kLeanTz32: PSV_BATCH_LEAN_FWD(32, 2) break;
+      case kLeanTz20: PSV_BATCH_LEAN(20, 3, 2) break;
+      case kLeanTz24: PSV_BATCH_LEAN(24, 2, 1) break;
+      case kLeanTz32: PSV_BATCH_LEAN(32, 2, 1) break;
       case kLeanTz4: PSV_BATCH_LEAN(4, 7, 5) break;
       case kLeanTz8W6: PSV_BATCH_LEAN(8, 6, 4) break;
       case kLeanTz12W5: PSV_BATCH_LEAN(12, 5, 3) break;
@@ -2604,7 +2596,6 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
       hipExtLaunchKernelGGL((batch_fused<8, 256>), g, t, 0, s, e0, e1, 0, b);
   }
 #undef PSV_BATCH_LEAN
-#undef PSV_BATCH_LEAN_FWD
   return hipGetLastError();
 }
 

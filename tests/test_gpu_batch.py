@@ -104,7 +104,7 @@ def test_batch_autotune_and_validation():
         batch.set_variant(2)
 
 
-@pytest.mark.parametrize("variant", [9, 0, 18, 22, 23])
+@pytest.mark.parametrize("variant", [9, 0, 18, 22, 23, 8, 13])
 def test_batch_gradients_equal_single_models(oracle_lib, variant):
     """Batched FWI gradients (forward with snapshots, adjoint with kernel
     correlation, shot by shot over all members) equal each member's own
