@@ -71,7 +71,9 @@ def _args():
                         "tiles (9 is the library default); 14 lean two "
                         "steps per launch (64x8); 15-17 lean with 2, 3, 4 "
                         "tiles of a strip per block; 18 lean 64x4; 19, 20 "
-                        "lean 64x8 / 64x12 at 6 / 5 waves per SIMD")
+                        "lean 64x8 / 64x12 at 6 / 5 waves per SIMD; 21 lean "
+                        "64x8 with halo columns by DMA; 22, 23 lean 64x8 / "
+                        "64x4 with a separate stress stage")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
@@ -129,15 +131,35 @@ def _args():
     return p.parse_args()
 
 
-def calibrate_copy(nbytes, iters=5):
-    """Mean time of a plain device copy of nbytes; returns (ms, GB/s of
-    read + write traffic)."""
+def calibrate_copy(nbytes, iters=5, runtime_copy=False):
+    """Mean time of a device copy of nbytes (the library's plain copy kernel,
+    or hipMemcpyAsync with runtime_copy); returns (ms, GB/s of read + write
+    traffic)."""
     import ctypes as C
     from psvwave_amd import _lib
     lib = _lib.load()
     ms = C.c_double()
-    _lib.check(lib.psv_calibrate_copy(int(nbytes), int(iters), C.byref(ms)))
+    _lib.check(lib.psv_calibrate_copy(int(nbytes), int(iters), int(bool(runtime_copy)),
+                                      C.byref(ms)))
     return ms.value, 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def stream_ceiling(launch_bytes, achieved_gbs):
+    """The device's copy rate -- read + write bytes over time, HIP events
+    around 20 copies, the better of the library's plain copy kernel and
+    hipMemcpyAsync device-to-device (the runtime's copy is faster on sizes
+    the Infinity Cache holds, the kernel on HBM-sized ones) -- with as many
+    bytes as one step launch moves and with 2 GB: the practical ceiling of
+    a streaming kernel beside the published peak."""
+    out = {}
+    for label, total in (("launch_bytes", launch_bytes), ("2GB", 2e9)):
+        nb = int(total / 2) // 16 * 16
+        kernel = calibrate_copy(nb, iters=20)[1]
+        runtime = calibrate_copy(nb, iters=20, runtime_copy=True)[1]
+        out[f"copy_GBs_at_{label}"] = round(max(kernel, runtime), 1)
+        out[f"copy_kernel_runtime_GBs_at_{label}"] = [round(kernel, 1), round(runtime, 1)]
+    out["frac_of_copy_at_launch_bytes"] = round(achieved_gbs / out["copy_GBs_at_launch_bytes"], 4)
+    return out
 
 
 def _host_cpu():
@@ -294,7 +316,8 @@ def monte_carlo(a):
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "note": "time = whole forward loops (HIP events), gaps included"},
+                     "note": "time = whole forward loops (HIP events), gaps included",
+                     "stream_ceiling": stream_ceiling(104.0 * points, achieved)},
         "misfit_mean": float(np.mean(mis)),
     }
 
@@ -766,6 +789,7 @@ def main():
     }
     if copy:
         out["roofline"]["copy_calibration"] = copy
+    out["roofline"]["stream_ceiling"] = stream_ceiling(bytes_per_launch, achieved)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.cpu_baseline_nt)
     print(json.dumps(out), flush=True)

@@ -253,10 +253,12 @@ int psv_device_bytes(const psv_model *m, int64_t *bytes);
 int psv_l2_misfit_noalloc(const double *syn_ux, const double *syn_uz,
                           const double *obs_ux, const double *obs_uz, int64_t n,
                           double dt, double *misfit, void *stream);
-/* Streaming copy of `bytes` (a multiple of 16) device-to-device with a plain
- * copy kernel, `iters` times; *ms is the mean kernel time (HIP events).
- * Reference point for byte counters and sustainable bandwidth. */
-int psv_calibrate_copy(int64_t bytes, int iters, double *ms);
+/* Streaming copy of `bytes` (a multiple of 16) device-to-device, `iters`
+ * times; *ms is the mean time per copy (HIP events around all of them).
+ * runtime_copy = 0: the library's plain copy kernel, the reference point for
+ * byte counters; 1: hipMemcpyAsync (the runtime's own copy), the practical
+ * streaming ceiling. */
+int psv_calibrate_copy(int64_t bytes, int iters, int runtime_copy, double *ms);
 
 /* ---- Batched Monte-Carlo forward runs (no reference counterpart: the
  * reference runs one fdModel at a time, README.md:16-17 states the

@@ -599,7 +599,7 @@ int psv_device_bytes(const psv_model *m, int64_t *bytes) {
   });
 }
 
-int psv_calibrate_copy(int64_t bytes, int iters, double *ms) {
+int psv_calibrate_copy(int64_t bytes, int iters, int runtime_copy, double *ms) {
   return guard([&] {
     need(ms, "ms");
     if (bytes <= 0 || bytes % 16 != 0 || iters < 1)
@@ -624,11 +624,15 @@ int psv_calibrate_copy(int64_t bytes, int iters, double *ms) {
       chk(hipMalloc((void **)&a, bytes), "hipMalloc");
       chk(hipMalloc((void **)&b, bytes), "hipMalloc");
       chk(psv::launch_fill(a, n, 1.0, s), "fill");
-      chk(psv::launch_copy(a, b, n, s), "copy");  // warm up
+      auto copy = [&]() {
+        return runtime_copy ? hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s)
+                            : psv::launch_copy(a, b, n, s);
+      };
+      chk(copy(), "copy");  // warm up
       chk(hipEventCreate(&e0), "hipEventCreate");
       chk(hipEventCreate(&e1), "hipEventCreate");
       chk(hipEventRecord(e0, s), "hipEventRecord");
-      for (int i = 0; i < iters; ++i) chk(psv::launch_copy(a, b, n, s), "copy");
+      for (int i = 0; i < iters; ++i) chk(copy(), "copy");
       chk(hipEventRecord(e1, s), "hipEventRecord");
       chk(hipEventSynchronize(e1), "hipEventSynchronize");
       float t = 0.f;
