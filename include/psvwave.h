@@ -281,6 +281,18 @@ int psv_batch_get_synthetic_data(const psv_batch *b, int m, double *ux,
                                  double *uz);
 /* Forward simulation of shot i_shot for every model (no snapshots). */
 int psv_batch_forward(psv_batch *b, int i_shot);
+/* Forward simulation of every shot of every model (what psv_batch_forward
+ * for shots 0 .. n_shots-1 does, bit for bit).  With two shots and a lean
+ * step variant both shots advance together, one launch per time step over
+ * every (tile, shot) of every model, the second shot in each model's second
+ * state buffer set (10 padded grids more per model; shot by shot when the
+ * device has no room for them, in graph mode and with the persistent
+ * loop). */
+int psv_batch_forward_all(psv_batch *b);
+/* *on = 1 when psv_batch_forward_all advances the shots together with the
+ * current variant and modes (allocating the second state sets if needed),
+ * else 0. */
+int psv_batch_lockstep(psv_batch *b, int *on);
 /* L2 misfit of every model (n values), as calculate_l2_misfit. */
 int psv_batch_misfits(psv_batch *b, double *out);
 /* Step kernel for all members: 0, 1, 6, 7 or 8-13 (see
@@ -302,9 +314,10 @@ int psv_batch_set_graph_mode(psv_batch *b, int on);
  * resident blocks meet at a grid barrier between steps, instead of one
  * launch per step (step variant 9 only; the same results). */
 int psv_batch_set_persistent(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22, 23 (ms_per_step[15],
- * -1 for a variant the batch cannot use) and keeps the fastest (*best).
- * Overwrites shot 0's synthetic data. */
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22, 23
+ * (ms_per_step[15]: ms per time step of one shot, -1 for a variant the batch
+ * cannot use) in the loop psv_batch_forward_all runs with each, and keeps
+ * the fastest (*best).  Overwrites the synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,
                        double *ms_per_step);
 /* Summed wall time of psv_batch_forward time loops (HIP events) and the

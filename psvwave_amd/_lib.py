@@ -125,6 +125,8 @@ _SIGNATURES = {
     "psv_batch_set_observed_data": [_P, _i, _D, _D],
     "psv_batch_get_synthetic_data": [_P, _i, _D, _D],
     "psv_batch_forward": [_P, _i],
+    "psv_batch_forward_all": [_P],
+    "psv_batch_lockstep": [_P, C.POINTER(_i)],
     "psv_batch_misfits": [_P, _D],
     "psv_batch_gradients": [_P],
     "psv_batch_set_persistent": [_P, _i],

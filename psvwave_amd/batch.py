@@ -76,8 +76,19 @@ class Batch:
         _lib.check(self._lib.psv_batch_forward(self._h, int(i_shot)))
 
     def forward_all(self):
-        for s in range(self.n_shots):
-            self.forward(s)
+        """Forward simulation of every shot for every member: bitwise what
+        forward(0), forward(1), ... give.  With two shots and a lean step
+        variant both shots advance in the same launches (each tile's two
+        shots side by side, sharing their material reads), the second shot
+        in a second state buffer set per member."""
+        _lib.check(self._lib.psv_batch_forward_all(self._h))
+
+    def lockstep(self):
+        """Whether forward_all advances the shots together with the current
+        variant and modes (allocates the second state sets if needed)."""
+        on = C.c_int()
+        _lib.check(self._lib.psv_batch_lockstep(self._h, C.byref(on)))
+        return bool(on.value)
 
     def misfits(self):
         """L2 misfit of every member (calculate_l2_misfit of each)."""
@@ -117,8 +128,10 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 
     def autotune(self, steps=20):
-        """Time variants 0, 1, 6, 7, 8-13, 18-20, 22, 23 over `steps` steps and keep the fastest.
-        Returns (best variant, {variant: ms per step})."""
+        """Time variants 0, 1, 6, 7, 8-13, 18-20, 22, 23 over `steps` steps,
+        each in the loop forward_all runs with it (both shots in lockstep
+        where that applies), and keep the fastest.  Returns (best variant,
+        {variant: ms per time step of one shot})."""
         best = C.c_int()
         ms = np.zeros(len(AUTOTUNE_VARIANTS))
         _lib.check(self._lib.psv_batch_autotune(self._h, int(steps), C.byref(best),

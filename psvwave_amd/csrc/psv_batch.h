@@ -38,6 +38,15 @@ class Batch {
   void set_persistent(bool on);
   // Forward simulation of shot i_shot for every member (no snapshots).
   void forward(int i_shot);
+  // Forward simulation of every shot of every member.  With two shots and
+  // a lean step variant (no graph capture, no persistent loop) both shots
+  // advance together: one launch per time step over every (tile, shot) of
+  // every member, the second shot in each member's second state buffer
+  // set, so the two reads of a tile's materials meet in L2.  Otherwise,
+  // or when the device has no room for the second state sets, shot by shot.
+  // Seismograms are those of forward(0), forward(1), ... bit for bit.
+  void forward_all();
+  bool lockstep_available();
   // Per-member L2 misfit over all shots (calculate_l2_misfit of each).
   void misfits(double *out);
   // FWI gradient of every member against its observed data (the reference's
@@ -50,7 +59,9 @@ class Batch {
   void gradients();
   // Times `steps` batched steps of each candidate variant (ms_per_variant
   // gets kBatchCandidates entries, -1 for a variant the batch cannot use),
-  // keeps the fastest; returns its time per step in ms.
+  // keeps the fastest; returns its time per step of one shot in ms.  Each
+  // variant is timed in the loop forward_all runs with it: both shots in
+  // lockstep where that applies, else shot 0 alone.
   double autotune(int steps, int *best, double *ms_per_variant);
   long long grid_points() const { return points_; }
   double loop_ms() const { return loop_ms_; }
@@ -59,6 +70,7 @@ class Batch {
  private:
   void build_tables();
   void enqueue_loop(int i_shot, int nsteps);
+  void enqueue_lockstep_loop(int nsteps);
   void clear_graphs();
   void clear_gradient_tables();
   void enqueue_gradient_loop(int i_shot, bool forward);
@@ -67,6 +79,8 @@ class Batch {
   int nt_ = 0, n_shots_ = 0, variant_ = kFusedTz8, xcd_remap_ = 1;
   hipStream_t stream_ = nullptr;
   StepArgs *descs_ = nullptr;  // [model][shot][parity]
+  StepArgs *ldescs_ = nullptr; // the same, shot s in state set s (forward_all)
+  bool no_lockstep_room_ = false;
   int2 *tiles_ = nullptr;      // (model, tile) pairs
   int ntiles_ = 0;
   long long points_ = 0;

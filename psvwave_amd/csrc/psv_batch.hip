@@ -44,6 +44,7 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   clear_graphs();
   if (descs_) (void)hipFree(descs_);
+  if (ldescs_) (void)hipFree(ldescs_);
   if (tiles_) (void)hipFree(tiles_);
   clear_gradient_tables();
   for (auto e : ev_)
@@ -116,8 +117,9 @@ void Batch::build_tables() {
   ntiles_ = (int)tiles.size();
   hip_check(hipStreamSynchronize(stream_), "sync");
   if (descs_) hip_check(hipFree(descs_), "hipFree");
+  if (ldescs_) hip_check(hipFree(ldescs_), "hipFree");
   if (tiles_) hip_check(hipFree(tiles_), "hipFree");
-  descs_ = nullptr;
+  descs_ = ldescs_ = nullptr;
   tiles_ = nullptr;
   hip_check(hipMalloc((void **)&descs_, std::max<size_t>(1, descs.size()) * sizeof(StepArgs)),
             "hipMalloc");
@@ -186,6 +188,54 @@ void Batch::forward(int i_shot) {
   } else {
     enqueue_loop(i_shot, nt_);
   }
+  hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
+  loop_ms_ += ms;
+}
+
+bool Batch::lockstep_available() {
+  if (n_shots_ != 2 || !step_is_lean(variant_) || graph_mode_ || persistent_ ||
+      no_lockstep_room_ || (long long)ntiles_ * n_shots_ > std::numeric_limits<int>::max())
+    return false;
+  if (ldescs_) return true;
+  try {
+    for (auto &m : models_) m->ensure_second_state();
+  } catch (const RuntimeError &) {
+    no_lockstep_room_ = true;  // keep what was allocated; run shot by shot
+    return false;
+  }
+  std::vector<StepArgs> descs;
+  for (auto &m : models_)
+    for (int s = 0; s < n_shots_; ++s)
+      for (int p = 0; p < 2; ++p) descs.push_back(m->forward_launch_args(s, p, s));
+  hip_check(hipMalloc((void **)&ldescs_, descs.size() * sizeof(StepArgs)), "hipMalloc");
+  hip_check(hipMemcpy(ldescs_, descs.data(), descs.size() * sizeof(StepArgs),
+                      hipMemcpyHostToDevice),
+            "upload");
+  return true;
+}
+
+// Zero both state sets of every member, then advance `nsteps` steps of
+// every shot with one launch per step (lockstep_available must hold).
+void Batch::enqueue_lockstep_loop(int nsteps) {
+  for (auto &m : models_)
+    for (int set = 0; set < n_shots_; ++set) m->enqueue_zero_state(stream_, set);
+  for (int k = 0; k < nsteps; ++k) {
+    BatchLaunch b{ldescs_, tiles_, ntiles_ * n_shots_, n_shots_, -1, k & 1,
+                  k, k + 1, xcd_remap_, 0, 1};
+    hip_check(launch_batch_step(variant_, false, b, stream_), "batch step");
+  }
+}
+
+void Batch::forward_all() {
+  if (!lockstep_available()) {
+    for (int s = 0; s < n_shots_; ++s) forward(s);
+    return;
+  }
+  hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+  enqueue_lockstep_loop(nt_);
   hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   float ms = 0.f;
@@ -278,14 +328,24 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
     if (ms_per_variant) ms_per_variant[k] = -1.0;
     if (step_is_lean(candidates[k]) && !lean) continue;
     set_variant(candidates[k], keep_remap);
-    enqueue_loop(0, std::min(steps, nt_));  // warm up
+    // the loop forward_all runs with this variant: both shots in lockstep
+    // where it can, else shot 0 alone; timed per step of one shot
+    const bool lock = lockstep_available();
+    const int n = std::min(steps, nt_);
+    auto run = [&]() {
+      if (lock)
+        enqueue_lockstep_loop(n);
+      else
+        enqueue_loop(0, n);
+    };
+    run();  // warm up
     hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-    enqueue_loop(0, std::min(steps, nt_));
+    run();
     hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "hipEventElapsedTime");
-    double per = ms / std::min(steps, nt_);
+    double per = ms / (n * (lock ? n_shots_ : 1));
     if (ms_per_variant) ms_per_variant[k] = per;
     if (per < best_ms) {
       best_ms = per;

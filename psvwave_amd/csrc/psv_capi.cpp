@@ -734,6 +734,17 @@ int psv_batch_forward(psv_batch *b, int i_shot) {
   return guard([&] { B(b).forward(i_shot); });
 }
 
+int psv_batch_forward_all(psv_batch *b) {
+  return guard([&] { B(b).forward_all(); });
+}
+
+int psv_batch_lockstep(psv_batch *b, int *on) {
+  return guard([&] {
+    need(on, "on");
+    *on = B(b).lockstep_available() ? 1 : 0;
+  });
+}
+
 int psv_batch_misfits(psv_batch *b, double *out) {
   return guard([&] {
     need(out, "out");

@@ -246,6 +246,13 @@ hipError_t launch_two_steps(int variant, const StepArgs &a, hipStream_t stream,
 // With `snapshots`, step it of a member stores (forward) or correlates with
 // (adjoint; launched with corr) snapshot slot it / si when si divides it:
 // the descriptor's snap pointers are the bases of a one-shot snapshot store.
+// One batched launch: descs[(member * n_shots + shot) * 2 + parity] are the
+// members' step arguments, tiles the (member, tile) pairs.  shot >= 0
+// advances that shot (ntiles = pairs); shot < 0 advances every shot at once
+// (lean steps): ntiles counts (pair, shot) items, item g being shot
+// g % n_shots of pair g / n_shots, so the shots of one tile are neighbours
+// in the grid (one XCD, close in time) and share their material reads in
+// its L2.
 struct BatchLaunch {
   const StepArgs *descs;
   const int2 *tiles;

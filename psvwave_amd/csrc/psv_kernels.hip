@@ -2128,8 +2128,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
   if (g >= b.ntiles) return;
-  const int2 mt = b.tiles[g];
-  StepArgs a = b.descs[((long long)mt.x * b.n_shots + b.shot) * 2 + b.parity];
+  const int all = b.shot < 0;
+  const int2 mt = b.tiles[all ? g / b.n_shots : g];
+  const int shot = all ? g % b.n_shots : b.shot;
+  StepArgs a = b.descs[((long long)mt.x * b.n_shots + shot) * 2 + b.parity];
   if (b.snapshots) batch_snapshot_slot(a, b);
   lean_tile<TZ, CORR, HALO_DMA, SEP>(a, mt.y, b.it);
 }

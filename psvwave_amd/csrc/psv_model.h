@@ -104,14 +104,17 @@ class Model {
   // Pieces of a forward time loop for batched launches across models
   // (psv_batch.hip): the arguments of a launch that reads state buffer set
   // `parity`, zeroing the state, and the receiver scan of two-step loops.
-  StepArgs forward_launch_args(int i_shot, int parity) const;
+  // state_set 1: the second state buffer set (ensure_second_state), so that
+  // two shots can advance side by side.
+  StepArgs forward_launch_args(int i_shot, int parity, int state_set = 0) const;
   // Batched gradients: the arguments of an adjoint launch reading state set
   // `parity` (adjoint sources of shot i_shot, kernel correlation), and a
   // snapshot store for one shot at a time ([slot][iz][ix] per field) whose
   // bases both launch kinds get in StepArgs::snap.
   StepArgs adjoint_launch_args(int i_shot, int parity) const;
   double *const *shot_snapshot_store();
-  void enqueue_zero_state(hipStream_t s) const;
+  void enqueue_zero_state(hipStream_t s, int state_set = 0) const;
+  void ensure_second_state();
   void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
   int tile_count() const { return ntiles_; }
   bool lean_ok() const { return lean_ok_; }

@@ -642,11 +642,7 @@ void Model::ensure_lane() {
   if (!ev_join_) hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming),
                            "hipEventCreate");
   if ((int)fwd_job_.size() < cfg_.n_shots) fwd_job_.assign(cfg_.n_shots, -1);
-  if (!lane_state_[0][0]) {
-    for (int p = 0; p < 2; ++p)
-      for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
-    sync();
-  }
+  ensure_second_state();
 }
 
 int Model::push_job(int lane, int kind, int shot, bool store, int dep0, int dep1) {
@@ -942,8 +938,10 @@ void Model::timed_loop(const std::function<void()> &enqueue) {
   timed_ += launches_ - before;
 }
 
-StepArgs Model::forward_launch_args(int i_shot, int parity) const {
+StepArgs Model::forward_launch_args(int i_shot, int parity, int state_set) const {
   check_shot(i_shot);
+  double *const(*state)[5] = state_set ? lane_state_ : state_;
+  if (!state[0][0]) throw RuntimeError("the second state buffer set is not allocated");
   StepArgs a = base_args();
   a.adjoint = 0;
   a.dts = cfg_.dt;
@@ -957,16 +955,16 @@ StepArgs Model::forward_launch_args(int i_shot, int parity) const {
   a.rec_off = adj_off_;
   a.rec_entries = adj_entries_;
   const int cur = parity & 1, nxt = 1 - cur;
-  a.vx_i = state_[cur][0];
-  a.vz_i = state_[cur][1];
-  a.txx_i = state_[cur][2];
-  a.tzz_i = state_[cur][3];
-  a.txz_i = state_[cur][4];
-  a.vx_o = state_[nxt][0];
-  a.vz_o = state_[nxt][1];
-  a.txx_o = state_[nxt][2];
-  a.tzz_o = state_[nxt][3];
-  a.txz_o = state_[nxt][4];
+  a.vx_i = state[cur][0];
+  a.vz_i = state[cur][1];
+  a.txx_i = state[cur][2];
+  a.tzz_i = state[cur][3];
+  a.txz_i = state[cur][4];
+  a.vx_o = state[nxt][0];
+  a.vz_o = state[nxt][1];
+  a.txx_o = state[nxt][2];
+  a.tzz_o = state[nxt][3];
+  a.txz_o = state[nxt][4];
   a.do_snap = 0;
   a.do_snap1 = 0;
   return a;
@@ -1001,10 +999,19 @@ double *const *Model::shot_snapshot_store() {
   return shot_snap_;
 }
 
-void Model::enqueue_zero_state(hipStream_t s) const {
+void Model::enqueue_zero_state(hipStream_t s, int state_set) const {
+  double *const(*state)[5] = state_set ? lane_state_ : state_;
   for (int k = 0; k < 2; ++k)
     for (int f = 0; f < 5; ++f)
-      hip_check(launch_fill(state_[k][f], npad_, 0.0, s), "zero state");
+      hip_check(launch_fill(state[k][f], npad_, 0.0, s), "zero state");
+}
+
+void Model::ensure_second_state() {
+  if (!lane_state_[0][0]) {
+    for (int p = 0; p < 2; ++p)
+      for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);
+    sync();
+  }
 }
 
 void Model::enqueue_receiver_scan(int i_shot, hipStream_t s) const {

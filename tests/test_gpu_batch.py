@@ -77,6 +77,35 @@ def test_batch_members_equal_single_models(oracle_lib, variant, graph, persisten
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("variant", [9, 10, 18, 22])
+def test_forward_all_lockstep_equals_shot_by_shot(variant):
+    """forward_all advances both shots in the same launches (second shot in a
+    second state buffer set); the seismograms equal forward(0), forward(1)
+    of the same batch bit for bit, on repeated calls too."""
+    import psvwave_amd
+    cfgs = _members()
+    batch = psvwave_amd.Batch(cfgs)
+    batch.set_variant(variant)
+    for m in range(len(cfgs)):
+        info = batch.infos[m]
+        batch.set_parameter_fields(m, *_fields(info["nx"], info["nz"], 50 + m))
+    print(f"[lockstep] variant {variant}", flush=True)
+    batch.forward(0)
+    batch.forward(1)
+    want = [batch.get_synthetic_data(m) for m in range(len(cfgs))]
+    assert batch.lockstep()
+    for rep in range(2):
+        batch.forward_all()
+        for m in range(len(cfgs)):
+            for a, b in zip(batch.get_synthetic_data(m), want[m]):
+                assert np.array_equal(a, b), (variant, m, rep)
+    batch.set_variant(0)  # fused steps: shot by shot
+    assert not batch.lockstep()
+    batch.set_variant(variant)
+    batch.set_graph_mode(True)
+    assert not batch.lockstep()
+
+
 def test_persistent_batch_loop_equals_single_models(oracle_lib):
     """One cooperative launch per shot's loop, a grid barrier between
     steps: the same results as per-step launches and as single models."""
