@@ -203,7 +203,11 @@ bool Batch::lockstep_available() {
   try {
     for (auto &m : models_) m->ensure_second_state();
   } catch (const RuntimeError &) {
-    no_lockstep_room_ = true;  // keep what was allocated; run shot by shot
+    // no room: keep what was allocated and run shot by shot; clear the
+    // failed allocation's error so that the next launch check does not
+    // report it
+    (void)hipGetLastError();
+    no_lockstep_room_ = true;
     return false;
   }
   std::vector<StepArgs> descs;
