@@ -45,7 +45,7 @@ def _measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (profiles/<kernel>_hbm_traffic.md, line `bytes_per_launch = N`),
     or None when no summary for this kernel exists."""
-    name = kernel.replace("<", "").replace(">", "").replace(",", "_")
+    name = kernel.replace("<", "").replace(">", "").replace(",", "_").replace(" ", "_")
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"{name}_hbm_traffic.md")
     try:
@@ -63,17 +63,20 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--variant", type=int, default=9,
+    p.add_argument("--variant", type=int, default=24,
                    help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split; "
                         "3-5 the same with reciprocal-multiply division; "
                         "6, 7 two steps per launch (64x8, 64x16); 8-13 lean "
                         "one-step kernels with 64 x 16, 8, 12, 20, 24, 32 "
-                        "tiles (9 is the library default); 14 lean two "
+                        "tiles; 14 lean two "
                         "steps per launch (64x8); 15-17 lean with 2, 3, 4 "
                         "tiles of a strip per block; 18 lean 64x4; 19, 20 "
                         "lean 64x8 / 64x12 at 6 / 5 waves per SIMD; 21 lean "
                         "64x8 with halo columns by DMA; 22, 23 lean 64x8 / "
-                        "64x4 with a separate stress stage")
+                        "64x4 with a separate stress stage; 24 (the library "
+                        "default) lean 64x8 with halo columns and ring "
+                        "operands by DMA, 6 waves per SIMD; 25 the same "
+                        "without the halo columns")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
@@ -724,6 +727,7 @@ def main():
                    18: "step_lean<4>", 19: "step_lean<8,6 waves>",
                    20: "step_lean<12,5 waves>", 21: "step_lean<8,halo DMA>",
                    22: "step_lean<8,separate stages>", 23: "step_lean<4,separate stages>",
+                   24: "step_lean<8,halo+ring DMA>", 25: "step_lean<8,ring DMA>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -212,7 +212,9 @@ enum StepVariant {
   kLeanTz8Halo = 21,
   kLeanTz8Sep = 22,
   kLeanTz4Sep = 23,
-  kNumVariants = 24
+  kLeanTz8Ring = 24,  // lean 64 x 8, halo columns and ring operands by DMA
+  kLeanTz8RingOnly = 25,  // lean 64 x 8, ring operands by DMA
+  kNumVariants = 26
 };
 constexpr int kMaxTileHeight = 32;
 

@@ -544,7 +544,11 @@ __device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
 // they are written as soon as computed (no barrier between the two uses of
 // one stage) and the velocity pass reads the start-of-step velocities back
 // from LDS instead of keeping them in registers.
-template <int TZ, bool CORR, bool HALO_DMA = false, bool SEP = false>
+// RDMA: the ring row's and ring columns' operands (old stresses, lm, mu)
+// also go straight into an LDS stage by DMA instead of waiting in
+// registers from the first load to the stress pass; with HALO_DMA this
+// frees enough registers for six waves per SIMD.
+template <int TZ, bool CORR, bool HALO_DMA = false, bool SEP = false, bool RDMA = false>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, const int it) {
   using L = Lean<TZ>;
   constexpr int R = L::R;
@@ -553,18 +557,25 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   // two stages side by side with SEP.
   constexpr int NV = 2 * L::VR * L::VW;
   constexpr int NS = TZ * L::XXW + L::ZZR * 64 + L::XZR * L::XZW;
-  constexpr int NI = SEP ? 16 : kInjStage;  // staged injection entries
+  constexpr int NI = SEP ? 16 : RDMA ? 32 : kInjStage;  // staged injection entries
   // the tile's taper factors: columns x0-2 .. x0+65, then rows z0-2 ..
   // z0+TZ+1, staged by DMA like the velocity rows, after the stages
   constexpr int TXN = 68, TZN = TZ + 4;
   constexpr int NST = SEP ? NV + NS : (NV > NS ? NV : NS);
-  __shared__ double lds[NST + TXN + TZN];
+  // RDMA ring stages: [wave][txz, tzz, lm, mu][64] for the ring rows, then
+  // [txx, txz, lm, mu][NCQ] for the ring-column items (two lanes per value
+  // in the transfers, so NCQ rounds NCI up to 32)
+  constexpr int NCQ = (L::NCI + 31) / 32 * 32;
+  constexpr int NRG = RDMA ? 4 * 4 * 64 + 4 * NCQ : 0;
+  __shared__ double lds[NST + TXN + TZN + NRG];
   double *const s_vx = lds, *const s_vz = lds + L::VR * L::VW;
   double *const s_xx = SEP ? lds + NV : lds, *const s_zz = s_xx + TZ * L::XXW;
   double *const s_xz = s_zz + L::ZZR * 64;
   double *const s_tap = lds + NST;
   lds_dbl *const d_vx = lds_ptr(lds), *const d_vz = d_vx + L::VR * L::VW;
   lds_dbl *const d_tap = d_vx + NST;
+  double *const s_gr = lds + NST + TXN + TZN, *const s_gc = s_gr + 4 * 4 * 64;
+  lds_dbl *const d_gr = d_tap + TXN + TZN, *const d_gc = d_gr + 4 * 4 * 64;
 
   // This tile's injection list bounds, before any memory access of the
   // kernel: a uniform load that nothing can have clobbered yet is a scalar
@@ -671,30 +682,61 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   }
   // this wave's ring row: only the fields its components read
   const int rr = w == 0 ? -2 : w == 1 ? -1 : w == 2 ? TZ : TZ + 1;
-  double gzz = 0.0, gxz = 0.0, glm = 0.0, gmu;
-  {
-    unsigned o = off(rr, lane);
-    if (w != 3) gxz = ldb(txzi, o);
-    if (w != 0) {
-      gzz = ldb(tzzi, o);
-      glm = ldb(lmp, o);
-    }
-    gmu = ldb(mup, o);
-  }
+  double gzz = 0.0, gxz = 0.0, glm = 0.0, gmu = 0.0;
   // ring-column items of this wave: item i = p * 64 + lane
   double cxx[L::NCP], cxz[L::NCP], clm[L::NCP], cmu[L::NCP];
 #pragma unroll
-  for (int p = 0; p < L::NCP; ++p) {
-    cxx[p] = cxz[p] = clm[p] = cmu[p] = 0.0;
-    const int i = p * 64 + lane;
-    if ((p & 3) == w && i < L::NCI) {
-      const int q = i / TZ, r = i - q * TZ;
-      const int c = q < 2 ? q - 2 : 62 + q;
-      unsigned o = off(r, c);
-      cxx[p] = ldb(txxi, o);
-      cxz[p] = ldb(txzi, o);
-      clm[p] = ldb(lmp, o);
-      cmu[p] = ldb(mup, o);
+  for (int p = 0; p < L::NCP; ++p) cxx[p] = cxz[p] = clm[p] = cmu[p] = 0.0;
+  if (RDMA) {
+    // ring row fields into s_gr[w][field], ring-column items (wave 3, two
+    // lanes per value: lane 2k + h moves dword h of item 32 m + k) into
+    // s_gc[field]
+    const unsigned o = off(rr, lane);
+    lds_dbl *const g = d_gr + w * 4 * 64;
+    if (w != 3) dma_row(txzi, o, g);
+    if (w != 0) {
+      dma_row(tzzi, o, g + 64);
+      dma_row(lmp, o, g + 128);
+    }
+    dma_row(mup, o, g + 192);
+    if (w == 3) {
+#pragma unroll
+      for (int m = 0; m < NCQ / 32; ++m) {
+        const int i = 32 * m + (lane >> 1);
+        if (i < L::NCI) {
+          const int q = i / TZ, r = i - q * TZ;
+          const int c = q < 2 ? q - 2 : 62 + q;
+          const unsigned oc = off(r, c) + (lane & 1) * 4u;
+          const double *const src[4] = {txxi, txzi, lmp, mup};
+#pragma unroll
+          for (int f = 0; f < 4; ++f)
+            __builtin_amdgcn_global_load_lds((const void *)((const char *)src[f] + oc),
+                                             (lds_void *)(d_gc + f * NCQ + 32 * m), 4, 0, 0);
+        }
+      }
+    }
+  } else {
+    {
+      unsigned o = off(rr, lane);
+      if (w != 3) gxz = ldb(txzi, o);
+      if (w != 0) {
+        gzz = ldb(tzzi, o);
+        glm = ldb(lmp, o);
+      }
+      gmu = ldb(mup, o);
+    }
+#pragma unroll
+    for (int p = 0; p < L::NCP; ++p) {
+      const int i = p * 64 + lane;
+      if ((p & 3) == w && i < L::NCI) {
+        const int q = i / TZ, r = i - q * TZ;
+        const int c = q < 2 ? q - 2 : 62 + q;
+        unsigned o = off(r, c);
+        cxx[p] = ldb(txxi, o);
+        cxz[p] = ldb(txzi, o);
+        clm[p] = ldb(lmp, o);
+        cmu[p] = ldb(mup, o);
+      }
     }
   }
   const int x = x0 + lane;
@@ -788,6 +830,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   }
   // ring row (its new values replace the loaded ones)
   {
+    const double *const g = s_gr + w * 4 * 64 + lane;  // RDMA stage
+    if (RDMA) {
+      if (w != 3) gxz = g[0];
+      if (w != 0) gzz = g[64], glm = g[128];
+      gmu = g[192];
+    }
     const int z = z0 + rr;
     if (xint && zint(z)) {
       const double t = fmin(tx, tap_z(rr));
@@ -796,11 +844,16 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
       const FastDiv gx{a.dx, a.rdx, a.zhx, a.zlx, &gf}, gz{a.dz, a.rdz, a.zhz, a.zlz, &gf};
       ring_row_stress<L::VW>(w, s_vx, s_vz, v, glm, gmu, t, dts, gx, gz, gzz, gxz);
       if (!gf.ok()) {
-        const unsigned o = opaque(off(rr, lane));
-        gzz = ldb(tzzi, o), gxz = ldb(txzi, o);
-        ring_row_stress<L::VW>(w, s_vx, s_vz, v, ldb(lmp, o), ldb(mup, o), t, dts, ix, iz,
-                               gzz, gxz);
-        __builtin_amdgcn_s_waitcnt(0);
+        if (RDMA) {
+          gzz = w != 0 ? g[64] : 0.0, gxz = w != 3 ? g[0] : 0.0;
+          ring_row_stress<L::VW>(w, s_vx, s_vz, v, g[128], g[192], t, dts, ix, iz, gzz, gxz);
+        } else {
+          const unsigned o = opaque(off(rr, lane));
+          gzz = ldb(tzzi, o), gxz = ldb(txzi, o);
+          ring_row_stress<L::VW>(w, s_vx, s_vz, v, ldb(lmp, o), ldb(mup, o), t, dts, ix, iz,
+                                 gzz, gxz);
+          __builtin_amdgcn_s_waitcnt(0);
+        }
       }
     }
   }
@@ -809,6 +862,10 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   for (int p = 0; p < L::NCP; ++p) {
     const int i = p * 64 + lane;
     if ((p & 3) == w && i < L::NCI) {
+      if (RDMA) {
+        cxx[p] = s_gc[i], cxz[p] = s_gc[NCQ + i];
+        clm[p] = s_gc[2 * NCQ + i], cmu[p] = s_gc[3 * NCQ + i];
+      }
       const int q = i / TZ, r = i - q * TZ;
       const int c = q < 2 ? q - 2 : 62 + q;
       const int xc = x0 + c, z = z0 + r;
@@ -820,11 +877,17 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
         lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), clm[p], cmu[p], t, dts, cx,
                                       cz, hxx, hzz, hxz);
         if (!cf.ok()) {
-          const unsigned o = opaque(off(r, c));
-          hxx = ldb(txxi, o), hxz = ldb(txzi, o);
-          lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), ldb(lmp, o), ldb(mup, o),
-                                        t, dts, ix, iz, hxx, hzz, hxz);
-          __builtin_amdgcn_s_waitcnt(0);
+          if (RDMA) {
+            hxx = s_gc[i], hxz = s_gc[NCQ + i];
+            lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), s_gc[2 * NCQ + i],
+                                          s_gc[3 * NCQ + i], t, dts, ix, iz, hxx, hzz, hxz);
+          } else {
+            const unsigned o = opaque(off(r, c));
+            hxx = ldb(txxi, o), hxz = ldb(txzi, o);
+            lean_stress<kXX | kXZ, L::VW>(s_vx, s_vz, L::V(r, c), ldb(lmp, o), ldb(mup, o),
+                                          t, dts, ix, iz, hxx, hzz, hxz);
+            __builtin_amdgcn_s_waitcnt(0);
+          }
         }
         cxx[p] = hxx, cxz[p] = hxz;
       }
@@ -938,12 +1001,13 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
 
 // CORR: instantiation for adjoint snapshot steps (kernel correlation; one
 // wave per SIMD fewer, for the correlation operands in flight).
-template <int TZ, int WPE, bool CORR, bool HALO_DMA = false, bool SEP = false>
+template <int TZ, int WPE, bool CORR, bool HALO_DMA = false, bool SEP = false,
+          bool RDMA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ, CORR, HALO_DMA, SEP>(a, tile, a.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA>(a, tile, a.it);
 }
 
 // ---------------------------------------------------------------------------
@@ -2123,7 +2187,8 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
   two_step_tile<TZ, NT>(a, mt.y);
 }
 
-template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false, bool SEP = false>
+template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false, bool SEP = false,
+          bool RDMA = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
@@ -2133,7 +2198,7 @@ batch_lean(BatchLaunch b) {
   const int shot = all ? g % b.n_shots : b.shot;
   StepArgs a = b.descs[((long long)mt.x * b.n_shots + shot) * 2 + b.parity];
   if (b.snapshots) batch_snapshot_slot(a, b);
-  lean_tile<TZ, CORR, HALO_DMA, SEP>(a, mt.y, b.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA>(a, mt.y, b.it);
 }
 
 // Persistent batched forward: one cooperative launch advances steps
@@ -2492,7 +2557,7 @@ int step_tile_height(int variant) {
 }
 bool step_is_lean(int variant) {
   return (variant >= kLeanTz16 && variant <= kLeanTz32) ||
-         (variant >= kLeanTz4 && variant <= kLeanTz4Sep);
+         (variant >= kLeanTz4 && variant <= kLeanTz8RingOnly);
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -2582,6 +2647,22 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
         else
           hipExtLaunchKernelGGL((batch_lean<4, 7, false, false, true>), g, t, 0, s, e0, e1, 0,
                                 b);
+        break;
+      case kLeanTz8Ring:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<8, 5, true, true, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<8, 6, false, true, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
+        break;
+      case kLeanTz8RingOnly:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<8, 5, true, false, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<8, 6, false, false, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
         break;
       default:
         return hipErrorInvalidValue;
@@ -2728,6 +2809,22 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
         hipExtLaunchKernelGGL((step_lean<4, 5, true, false, true>), g, b, 0, s, e0, e1, 0, a);
       else
         hipExtLaunchKernelGGL((step_lean<4, 7, false, false, true>), g, b, 0, s, e0, e1, 0, a);
+      break;
+    case kLeanTz8Ring:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 5, true, true, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 6, false, true, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
+      break;
+    case kLeanTz8RingOnly:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 5, true, false, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 6, false, false, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
       break;
     case kLeanPipe2Tz8:
     case kLeanPipe3Tz8:

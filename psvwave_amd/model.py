@@ -384,9 +384,16 @@ class fdModel:
         return stf, mom, taper
 
     def set_step_variant(self, variant, xcd_remap=True):
-        """Step kernel: 0 fused 64x8 tiles, 1 fused 64x16, 2 split (two
-        launches per step); 3-5 the same with reciprocal-multiply division
-        by the grid spacings (bit-identical results)."""
+        """Step kernel (bit-identical results): 0 fused 64x8 tiles, 1 fused
+        64x16, 2 split (two launches per step); 3-5 the same with
+        reciprocal-multiply division by the grid spacings; 6, 7 two steps per
+        launch; 8-13 lean kernels with 64 x 16, 8, 12, 20, 24, 32 tiles; 14
+        lean two steps per launch; 15-17 lean strips of 2-4
+        tiles per block; 18 lean 64x4; 19, 20 lean 64x8 / 64x12 at 6 / 5
+        waves per SIMD; 21 lean 64x8 with halo columns by DMA; 22, 23 lean
+        64x8 / 64x4 with separate stress stages; 24 (the default) lean 64x8
+        with halo columns and ring operands by DMA (6 waves per SIMD); 25
+        the same without the halo columns."""
         _lib.check(self._lib.psv_set_step_variant(self._h, int(variant),
                                                   int(bool(xcd_remap))))
 

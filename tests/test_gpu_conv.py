@@ -293,7 +293,7 @@ def test_unet_training_gradients_match_torch():
             rel = float((got - ref).norm() / ref.norm())
             assert rel < 0.1, (names[mod], rel)
     rel, rel16 = (tot / totref) ** 0.5, (tot16 / totref) ** 0.5
-    assert abs(float(loss) - float(ref_loss)) < 0.02 * float(ref_loss)
+    assert abs(float(loss.detach()) - float(ref_loss.detach())) < 0.02 * float(ref_loss.detach())
     assert rel < 3 * rel16 + 0.01, (rel, rel16)
 
 
