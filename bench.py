@@ -76,7 +76,8 @@ def _args():
                         "64x4 with a separate stress stage; 24 (the library "
                         "default) lean 64x8 with halo columns and ring "
                         "operands by DMA, 6 waves per SIMD; 25 the same "
-                        "without the halo columns")
+                        "without the halo columns; 26 lean 64x16 with ring "
+                        "operands by DMA, 4 waves per SIMD")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
@@ -728,6 +729,7 @@ def main():
                    20: "step_lean<12,5 waves>", 21: "step_lean<8,halo DMA>",
                    22: "step_lean<8,separate stages>", 23: "step_lean<4,separate stages>",
                    24: "step_lean<8,halo+ring DMA>", 25: "step_lean<8,ring DMA>",
+                   26: "step_lean<16,ring DMA>",
                    }[a.variant]
     out = {
         "metric": METRIC,

@@ -211,7 +211,9 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * 64 x 4 lean kernels with separate velocity and stress LDS stages, 24 the
  * 64 x 8 lean kernel with halo columns and ring-point operands moved
  * straight into LDS (six waves per SIMD without spills; the default when
- * dx, dz lie in [2^-40, 2^40]), 25 the same without the halo columns.
+ * dx, dz lie in [2^-40, 2^40]), 25 the same without the halo columns, 26
+ * the 64 x 16 lean kernel with ring-point operands in LDS (four waves per
+ * SIMD).
  * All variants give bitwise identical results.  xcd_remap: 1 keeps
  * neighbouring tiles on one XCD.
  * timing_stride k > 0 records the kernel start and end of every k-th step
@@ -297,7 +299,7 @@ int psv_batch_forward_all(psv_batch *b);
 int psv_batch_lockstep(psv_batch *b, int *on);
 /* L2 misfit of every model (n values), as calculate_l2_misfit. */
 int psv_batch_misfits(psv_batch *b, double *out);
-/* Step kernel for all members: 0, 1, 6, 7, 8-13 or 18-24 (see
+/* Step kernel for all members: 0, 1, 6, 7, 8-13 or 18-26 (see
  * psv_set_step_variant). */
 int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
 /* FWI gradient of every member against its observed data (what
@@ -306,7 +308,7 @@ int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
  * with kernel correlation -- one launch per time step over every member --
  * then misfits (psv_batch_misfits) and velocity kernels
  * (psv_batch_get_kernels, (nx, nz) each).  Needs one snapshot_interval for
- * all members, dividing nt, and step variant 0, 1, 8-13 or 18-24; each
+ * all members, dividing nt, and step variant 0, 1, 8-13 or 18-26; each
  * member keeps one shot's snapshots.  Results equal each member's run_model bit for bit. */
 int psv_batch_gradients(psv_batch *b);
 int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
@@ -316,8 +318,8 @@ int psv_batch_set_graph_mode(psv_batch *b, int on);
  * resident blocks meet at a grid barrier between steps, instead of one
  * launch per step (step variant 9 only; the same results). */
 int psv_batch_set_persistent(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22-24
- * (ms_per_step[16]: ms per time step of one shot, -1 for a variant the batch
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22-24, 26
+ * (ms_per_step[17]: ms per time step of one shot, -1 for a variant the batch
  * cannot use) in the loop psv_batch_forward_all runs with each, and keeps
  * the fastest (*best).  Overwrites the synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,

@@ -89,7 +89,7 @@ void Batch::clear_gradient_tables() {
 void Batch::set_variant(int variant, int xcd_remap) {
   if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
       variant != kTwoStepTz16 && !step_is_lean(variant))
-    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8-13 or 18-24");
+    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8-13 or 18-26");
   if (step_is_lean(variant))
     for (auto &m : models_)
       if (!m->lean_ok())
@@ -265,7 +265,7 @@ void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
 
 void Batch::gradients() {
   if (!batch_gradient_variant_ok(variant_))
-    throw ValueError("batched gradients need step variant 0, 1, 8-13 or 18-24");
+    throw ValueError("batched gradients need step variant 0, 1, 8-13 or 18-26");
   si_ = models_[0]->config().snapshot_interval;
   for (auto &m : models_)
     if (m->config().snapshot_interval != si_)
@@ -322,7 +322,8 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
   const int candidates[kBatchCandidates] = {
       kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16, kLeanTz16,
       kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32,
-      kLeanTz4,  kLeanTz8W6, kLeanTz12W5, kLeanTz8Sep, kLeanTz4Sep, kLeanTz8Ring};
+      kLeanTz4,  kLeanTz8W6, kLeanTz12W5, kLeanTz8Sep, kLeanTz4Sep, kLeanTz8Ring,
+      kLeanTz16Ring};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;

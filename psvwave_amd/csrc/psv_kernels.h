@@ -214,7 +214,8 @@ enum StepVariant {
   kLeanTz4Sep = 23,
   kLeanTz8Ring = 24,  // lean 64 x 8, halo columns and ring operands by DMA
   kLeanTz8RingOnly = 25,  // lean 64 x 8, ring operands by DMA
-  kNumVariants = 26
+  kLeanTz16Ring = 26,     // lean 64 x 16, ring operands by DMA (4 waves per SIMD)
+  kNumVariants = 27
 };
 constexpr int kMaxTileHeight = 32;
 

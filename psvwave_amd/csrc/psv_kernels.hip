@@ -2537,6 +2537,7 @@ int step_tile_height(int variant) {
     case kFusedTz16Recip:
     case kTwoStepTz16:
     case kLeanTz16:
+    case kLeanTz16Ring:
       return 16;
     case kLeanTz12:
       return 12;
@@ -2557,7 +2558,7 @@ int step_tile_height(int variant) {
 }
 bool step_is_lean(int variant) {
   return (variant >= kLeanTz16 && variant <= kLeanTz32) ||
-         (variant >= kLeanTz4 && variant <= kLeanTz8RingOnly);
+         (variant >= kLeanTz4 && variant <= kLeanTz16Ring);
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -2662,6 +2663,14 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
                                 e1, 0, b);
         else
           hipExtLaunchKernelGGL((batch_lean<8, 6, false, false, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
+        break;
+      case kLeanTz16Ring:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<16, 3, true, false, false, true>), g, t, 0, s, e0,
+                                e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<16, 4, false, false, false, true>), g, t, 0, s, e0,
                                 e1, 0, b);
         break;
       default:
@@ -2824,6 +2833,14 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
                               0, a);
       else
         hipExtLaunchKernelGGL((step_lean<8, 6, false, false, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
+      break;
+    case kLeanTz16Ring:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<16, 3, true, false, false, true>), g, b, 0, s, e0, e1,
+                              0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<16, 4, false, false, false, true>), g, b, 0, s, e0, e1,
                               0, a);
       break;
     case kLeanPipe2Tz8:

@@ -393,7 +393,8 @@ class fdModel:
         waves per SIMD; 21 lean 64x8 with halo columns by DMA; 22, 23 lean
         64x8 / 64x4 with separate stress stages; 24 (the default) lean 64x8
         with halo columns and ring operands by DMA (6 waves per SIMD); 25
-        the same without the halo columns."""
+        the same without the halo columns; 26 lean 64x16 with ring operands
+        by DMA (4 waves per SIMD)."""
         _lib.check(self._lib.psv_set_step_variant(self._h, int(variant),
                                                   int(bool(xcd_remap))))
 
