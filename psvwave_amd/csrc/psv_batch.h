@@ -71,6 +71,12 @@ class Batch {
   void build_tables();
   void enqueue_loop(int i_shot, int nsteps);
   void enqueue_lockstep_loop(int nsteps);
+  // every member's state buffer set `set` zeroed in one launch
+  void ensure_zero_table(int set);
+  void enqueue_zero_states(int set);
+  double **zero_ptrs_[2] = {nullptr, nullptr};
+  long long *zero_lens_ = nullptr;
+  long long zero_max_ = 0;
   void clear_graphs();
   void clear_gradient_tables();
   void enqueue_gradient_loop(int i_shot, bool forward);

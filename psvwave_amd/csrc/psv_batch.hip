@@ -46,6 +46,9 @@ Batch::~Batch() {
   if (descs_) (void)hipFree(descs_);
   if (ldescs_) (void)hipFree(ldescs_);
   if (tiles_) (void)hipFree(tiles_);
+  for (auto p : zero_ptrs_)
+    if (p) (void)hipFree(p);
+  if (zero_lens_) (void)hipFree(zero_lens_);
   clear_gradient_tables();
   for (auto e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -131,13 +134,49 @@ void Batch::build_tables() {
   hip_check(hipMemcpy(tiles_, tiles.data(), tiles.size() * sizeof(int2),
                       hipMemcpyHostToDevice),
             "upload");
+  ensure_zero_table(0);
+}
+
+// The device tables of every member's state set `set` (built before any
+// stream capture: the allocation and upload may not be captured).
+void Batch::ensure_zero_table(int set) {
+  if (!zero_ptrs_[set]) {
+    std::vector<double *> ptrs;
+    std::vector<long long> lens;
+    for (auto &m : models_) {
+      m->state_arrays(set, ptrs);
+      lens.resize(ptrs.size(), m->padded_points());
+      zero_max_ = std::max(zero_max_, m->padded_points());
+    }
+    for (double *p : ptrs)
+      if (!p) throw RuntimeError("a state buffer set is not allocated");
+    hip_check(hipMalloc((void **)&zero_ptrs_[set], ptrs.size() * sizeof(double *)),
+              "hipMalloc");
+    hip_check(hipMemcpy(zero_ptrs_[set], ptrs.data(), ptrs.size() * sizeof(double *),
+                        hipMemcpyHostToDevice),
+              "upload");
+    if (!zero_lens_) {
+      hip_check(hipMalloc((void **)&zero_lens_, lens.size() * sizeof(long long)), "hipMalloc");
+      hip_check(hipMemcpy(zero_lens_, lens.data(), lens.size() * sizeof(long long),
+                          hipMemcpyHostToDevice),
+                "upload");
+    }
+  }
+}
+
+void Batch::enqueue_zero_states(int set) {
+  if (!zero_ptrs_[set]) throw RuntimeError("state zeroing table missing");
+  const int n = 10 * size();
+  const int bx = (int)std::min<long long>(64, (zero_max_ + 2047) / 2048);
+  hip_check(launch_zero_many(zero_ptrs_[set], zero_lens_, n, std::max(1, bx), stream_),
+            "zero states");
 }
 
 // Zero every member's state, then advance `nsteps` steps of shot i_shot
 // with one launch per step (or per pair of steps), then form the receiver
 // running sums when the two-step kernels recorded increments.
 void Batch::enqueue_loop(int i_shot, int nsteps) {
-  for (auto &m : models_) m->enqueue_zero_state(stream_);
+  enqueue_zero_states(0);
   if (persistent_) {
     BatchLaunch b{descs_, tiles_, ntiles_, n_shots_, i_shot, 0, 0, 0, xcd_remap_, 0, 1};
     hip_check(launch_batch_persistent(b, 0, nsteps, stream_), "persistent batch loop");
@@ -210,6 +249,7 @@ bool Batch::lockstep_available() {
     no_lockstep_room_ = true;
     return false;
   }
+  ensure_zero_table(1);
   std::vector<StepArgs> descs;
   for (auto &m : models_)
     for (int s = 0; s < n_shots_; ++s)
@@ -224,8 +264,7 @@ bool Batch::lockstep_available() {
 // Zero both state sets of every member, then advance `nsteps` steps of
 // every shot with one launch per step (lockstep_available must hold).
 void Batch::enqueue_lockstep_loop(int nsteps) {
-  for (auto &m : models_)
-    for (int set = 0; set < n_shots_; ++set) m->enqueue_zero_state(stream_, set);
+  for (int set = 0; set < n_shots_; ++set) enqueue_zero_states(set);
   for (int k = 0; k < nsteps; ++k) {
     BatchLaunch b{ldescs_, tiles_, ntiles_ * n_shots_, n_shots_, -1, k & 1,
                   k, k + 1, xcd_remap_, 0, 1};
@@ -251,7 +290,7 @@ void Batch::forward_all() {
 // forward storing each member's snapshots of this shot, or adjoint (time
 // steps nt-1 .. 0) correlating with them.
 void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
-  for (auto &m : models_) m->enqueue_zero_state(stream_);
+  enqueue_zero_states(0);
   const StepArgs *descs = forward ? gfwd_ : gadj_;
   for (int k = 0; k < nt_; ++k) {
     const int it = forward ? k : nt_ - 1 - k;

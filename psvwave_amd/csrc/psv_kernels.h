@@ -320,6 +320,10 @@ hipError_t launch_receiver_scan(double *rx, double *rz, int nr, int nt,
                                 hipStream_t s);
 
 hipError_t launch_fill(double *p, long long n, double v, hipStream_t s);
+// Zeroes arrays ptrs[k] of lens[k] doubles, k < n (device tables), in one
+// launch of blocks_x x n blocks.
+hipError_t launch_zero_many(double *const *ptrs, const long long *lens, int n, int blocks_x,
+                            hipStream_t s);
 hipError_t launch_update_from_velocity(const double *vp, const double *vs,
                                        const double *rho, double *lm,
                                        double *mu, double *b, int nx, int nz,

@@ -2296,6 +2296,16 @@ __global__ void __launch_bounds__(kBlock) step_split_velocity(StepArgs a) {
   }
 }
 
+// Zeroes n arrays at once: blockIdx.y picks the array, the blocks along x
+// stride over it.
+__global__ void zero_many_kernel(double *const *ptrs, const long long *lens) {
+  double *const p = ptrs[blockIdx.y];
+  const long long n = lens[blockIdx.y];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    p[i] = 0.0;
+}
+
 __global__ void fill_kernel(double *p, long long n, double v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
@@ -2906,6 +2916,13 @@ hipError_t launch_replay_correlation(double *rho_k, double *lam_k, double *mu_k,
   if (n <= 0 || nslots <= 0) return hipSuccess;
   hipLaunchKernelGGL(replay_correlation_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      rho_k, lam_k, mu_k, d, nslots, ngrid, nx, kx0, kx1, kz0, kz1);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_many(double *const *ptrs, const long long *lens, int n, int blocks_x,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_many_kernel, dim3(blocks_x, n), dim3(256), 0, s, ptrs, lens);
   return hipGetLastError();
 }
 

@@ -114,6 +114,10 @@ class Model {
   StepArgs adjoint_launch_args(int i_shot, int parity) const;
   double *const *shot_snapshot_store();
   void enqueue_zero_state(hipStream_t s, int state_set = 0) const;
+  // The ten padded arrays of a state buffer set (2 parities x 5 fields) and
+  // their length, for batched zeroing.
+  void state_arrays(int state_set, std::vector<double *> &out) const;
+  long long padded_points() const { return npad_; }
   void ensure_second_state();
   void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
   int tile_count() const { return ntiles_; }

@@ -1006,6 +1006,12 @@ void Model::enqueue_zero_state(hipStream_t s, int state_set) const {
       hip_check(launch_fill(state[k][f], npad_, 0.0, s), "zero state");
 }
 
+void Model::state_arrays(int state_set, std::vector<double *> &out) const {
+  double *const(*state)[5] = state_set ? lane_state_ : state_;
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 5; ++f) out.push_back(state[k][f]);
+}
+
 void Model::ensure_second_state() {
   if (!lane_state_[0][0]) {
     for (int p = 0; p < 2; ++p)
