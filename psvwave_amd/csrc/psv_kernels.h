@@ -64,6 +64,10 @@ struct StepArgs {
   double *snap[5];  // vx, vz, txx, tzz, txz
   // kernel correlation (adjoint, do_snap steps): Lame kernels, compact
   double *lam_k, *mu_k, *rho_k;
+  // kfresh: the three arrays start at zero for this step (a deferred
+  // adjoint's per-slot terms), so the correlation writes its terms without
+  // reading them (0 - e == -e, 0 + e == e exactly)
+  int kfresh;
   int kx0, kx1, kz0, kz1;
   double wcorr;  // snapshot_interval * dt
   // injection lists for this shot, CSR by tile

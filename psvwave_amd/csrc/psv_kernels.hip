@@ -135,7 +135,8 @@ __device__ inline void owner_snapshot(const StepArgs &a, double *const *snap,
     snap[4][ci] = txz;
   } else if (x >= a.kx0 && x < a.kx1 && z >= a.kz0 && z < a.kz1) {
     long long pi = pidx(a, x, z);
-    double rk = a.rho_k[ci], lk = a.lam_k[ci], mk = a.mu_k[ci];
+    double rk = 0.0, lk = 0.0, mk = 0.0;
+    if (!a.kfresh) rk = a.rho_k[ci], lk = a.lam_k[ci], mk = a.mu_k[ci];
     correlate_point(a.wcorr, a.lm[pi], a.mu[pi], vx, vz, txx, tzz, txz,
                     snap[0][ci], snap[1][ci], snap[2][ci], snap[3][ci],
                     snap[4][ci], rk, lk, mk);
@@ -674,9 +675,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
         const long long ci = (long long)z * a.nx + xx;
 #pragma unroll
         for (int f = 0; f < 5; ++f) csv[j][f] = a.snap[f][ci];
-        crk[j] = a.rho_k[ci];
-        clk[j] = a.lam_k[ci];
-        cmk[j] = a.mu_k[ci];
+        crk[j] = clk[j] = cmk[j] = 0.0;
+        if (!a.kfresh) {
+          crk[j] = a.rho_k[ci];
+          clk[j] = a.lam_k[ci];
+          cmk[j] = a.mu_k[ci];
+        }
       }
     }
   }
@@ -1096,9 +1100,12 @@ __device__ __forceinline__ void lean_fetch_tile(const StepArgs &a, int tile, int
         const long long ci = (long long)z * a.nx + x;
 #pragma unroll
         for (int f = 0; f < 5; ++f) g.csv[j][f] = a.snap[f][ci];
-        g.crk[j] = a.rho_k[ci];
-        g.clk[j] = a.lam_k[ci];
-        g.cmk[j] = a.mu_k[ci];
+        g.crk[j] = g.clk[j] = g.cmk[j] = 0.0;
+        if (!a.kfresh) {
+          g.crk[j] = a.rho_k[ci];
+          g.clk[j] = a.lam_k[ci];
+          g.cmk[j] = a.mu_k[ci];
+        }
       }
     }
   }

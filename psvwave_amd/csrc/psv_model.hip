@@ -529,13 +529,18 @@ int Model::enqueue_range(StepArgs &a, double *(*st)[5], int parity,
     a.do_snap = snaps && (it % si == 0) && slot(it, a.snap);
     a.it1 = it1;
     a.do_snap1 = pair && snaps && (it1 % si == 0) && slot(it1, a.snap1);
+    a.kfresh = 0;
     if (defer && a.do_snap && !forward) {
       // deferred correlation (one-step kernels only): this snapshot slot's
-      // terms go to their own zeroed slice instead of the kernels
+      // terms go to their own slice instead of the kernels, written without
+      // being read (as if the slice started at zero), so the slice needs no
+      // clearing; the replay reads only the kernel region, which every
+      // correlation step writes in full
       const long long off = (long long)(it / si) * 3 * ngrid_;
       a.rho_k = defer + off;
       a.lam_k = defer + off + ngrid_;
       a.mu_k = defer + off + 2 * ngrid_;
+      a.kfresh = 1;
     }
     const int covered = pair ? 2 : 1;
     bool timed = pairs && timing_stride_ > 0 && (launches_ / covered) % timing_stride_ == 0;
@@ -706,8 +711,6 @@ void Model::pump(bool all) {
           for (int p = 0; p < 2; ++p)
             for (int f = 0; f < 5; ++f)
               hip_check(launch_fill(st[p][f], npad_, 0.0, s), "zero state");
-          if (jb.kind == kJobDeferred)
-            hip_check(launch_fill(defer_, defer_n_, 0.0, s), "zero deferred terms");
         }
       }
       if (jb.k < nt) {
@@ -1397,8 +1400,6 @@ void Model::run_model_two_lanes(bool adj) {
                                            (long long)cfg_.nr * nt, ln.s),
                     "adjoint sources");
           ln.a = adjoint_args(shot);
-          if (jb.kind == kAdjDeferred)
-            hip_check(launch_fill(defer_, defer_n_, 0.0, ln.s), "zero deferred terms");
         }
         ln.a.rec_increments = (fwd && two) ? 1 : 0;
         ln.a.rec_off = adj_off_;
