@@ -63,7 +63,7 @@ def _args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--variant", type=int, default=24,
+    p.add_argument("--variant", type=int, default=27,
                    help="step kernel: 0 fused 64x8, 1 fused 64x16, 2 split; "
                         "3-5 the same with reciprocal-multiply division; "
                         "6, 7 two steps per launch (64x8, 64x16); 8-13 lean "
@@ -73,11 +73,13 @@ def _args():
                         "tiles of a strip per block; 18 lean 64x4; 19, 20 "
                         "lean 64x8 / 64x12 at 6 / 5 waves per SIMD; 21 lean "
                         "64x8 with halo columns by DMA; 22, 23 lean 64x8 / "
-                        "64x4 with a separate stress stage; 24 (the library "
-                        "default) lean 64x8 with halo columns and ring "
+                        "64x4 with a separate stress stage; 24 lean 64x8 "
+                        "with halo columns and ring "
                         "operands by DMA, 6 waves per SIMD; 25 the same "
                         "without the halo columns; 26 lean 64x16 with ring "
-                        "operands by DMA, 4 waves per SIMD")
+                        "operands by DMA, 4 waves per SIMD; 27 (the library "
+                        "default), 28 variants 24, 26 with loads capped to "
+                        "the grid")
     p.add_argument("--no-xcd-remap", action="store_true")
     p.add_argument("--sequential", action="store_true",
                    help="drive the shots call by call (forward_simulate per "
@@ -730,6 +732,8 @@ def main():
                    22: "step_lean<8,separate stages>", 23: "step_lean<4,separate stages>",
                    24: "step_lean<8,halo+ring DMA>", 25: "step_lean<8,ring DMA>",
                    26: "step_lean<16,ring DMA>",
+                   27: "step_lean<8,halo+ring DMA,capped>",
+                   28: "step_lean<16,ring DMA,capped>",
                    }[a.variant]
     out = {
         "metric": METRIC,

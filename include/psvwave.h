@@ -210,10 +210,12 @@ int psv_load_model(psv_model *m, const char *de_path, const char *vp_path,
  * lean kernel with halo columns in the row transfers, 22 / 23 the 64 x 8 /
  * 64 x 4 lean kernels with separate velocity and stress LDS stages, 24 the
  * 64 x 8 lean kernel with halo columns and ring-point operands moved
- * straight into LDS (six waves per SIMD without spills; the default when
- * dx, dz lie in [2^-40, 2^40]), 25 the same without the halo columns, 26
+ * straight into LDS (six waves per SIMD without spills), 25 the same
+ * without the halo columns, 26
  * the 64 x 16 lean kernel with ring-point operands in LDS (four waves per
- * SIMD).
+ * SIMD), 27 / 28 variants 24 / 26 with their loads capped to the grid (the
+ * tiles on the right and bottom edges read no padding; 27 is the default
+ * when dx, dz lie in [2^-40, 2^40]).
  * All variants give bitwise identical results.  xcd_remap: 1 keeps
  * neighbouring tiles on one XCD.
  * timing_stride k > 0 records the kernel start and end of every k-th step
@@ -299,7 +301,7 @@ int psv_batch_forward_all(psv_batch *b);
 int psv_batch_lockstep(psv_batch *b, int *on);
 /* L2 misfit of every model (n values), as calculate_l2_misfit. */
 int psv_batch_misfits(psv_batch *b, double *out);
-/* Step kernel for all members: 0, 1, 6, 7, 8-13 or 18-26 (see
+/* Step kernel for all members: 0, 1, 6, 7, 8-13 or 18-28 (see
  * psv_set_step_variant). */
 int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
 /* FWI gradient of every member against its observed data (what
@@ -308,7 +310,7 @@ int psv_batch_set_variant(psv_batch *b, int variant, int xcd_remap);
  * with kernel correlation -- one launch per time step over every member --
  * then misfits (psv_batch_misfits) and velocity kernels
  * (psv_batch_get_kernels, (nx, nz) each).  Needs one snapshot_interval for
- * all members, dividing nt, and step variant 0, 1, 8-13 or 18-26; each
+ * all members, dividing nt, and step variant 0, 1, 8-13 or 18-28; each
  * member keeps one shot's snapshots.  Results equal each member's run_model bit for bit. */
 int psv_batch_gradients(psv_batch *b);
 int psv_batch_get_kernels(const psv_batch *b, int m, double *vp_k, double *vs_k,
@@ -318,8 +320,8 @@ int psv_batch_set_graph_mode(psv_batch *b, int on);
  * resident blocks meet at a grid barrier between steps, instead of one
  * launch per step (step variant 9 only; the same results). */
 int psv_batch_set_persistent(psv_batch *b, int on);
-/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22-24, 26
- * (ms_per_step[17]: ms per time step of one shot, -1 for a variant the batch
+/* Times `steps` batched steps of variants 0, 1, 6, 7, 8-13, 18-20, 22-24,
+ * 26-28 (ms_per_step[19]: ms per time step of one shot, -1 for a variant the batch
  * cannot use) in the loop psv_batch_forward_all runs with each, and keeps
  * the fastest (*best).  Overwrites the synthetic data. */
 int psv_batch_autotune(psv_batch *b, int steps, int *best,

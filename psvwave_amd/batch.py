@@ -14,7 +14,8 @@ from . import _lib
 from .model import _f64
 
 # step variants Batch.autotune times, in the library's order
-AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23, 24, 26)
+AUTOTUNE_VARIANTS = (0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23, 24, 26, 27,
+                     28)
 
 
 class Batch:
@@ -128,7 +129,7 @@ class Batch:
         _lib.check(self._lib.psv_batch_set_graph_mode(self._h, int(bool(on))))
 
     def autotune(self, steps=20):
-        """Time variants 0, 1, 6, 7, 8-13, 18-20, 22-24, 26 over `steps` steps,
+        """Time variants 0, 1, 6, 7, 8-13, 18-20, 22-24, 26-28 over `steps` steps,
         each in the loop forward_all runs with it (both shots in lockstep
         where that applies), and keep the fastest.  Returns (best variant,
         {variant: ms per time step of one shot})."""

@@ -11,8 +11,8 @@
 namespace psv {
 
 // Variants autotune() times, in this order: 0, 1, 6, 7, 8, 9, 10, 11, 12, 13,
-// 18, 19, 20, 22, 23, 24, 26.
-constexpr int kBatchCandidates = 17;
+// 18, 19, 20, 22, 23, 24, 26, 27, 28.
+constexpr int kBatchCandidates = 19;
 
 class Batch {
  public:

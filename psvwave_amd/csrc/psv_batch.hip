@@ -36,7 +36,7 @@ Batch::Batch(const std::vector<Config> &cfgs) {
   for (auto &m : models_) points_ += m->grid_points();
   bool lean = true;
   for (auto &m : models_) lean = lean && m->lean_ok();
-  if (lean) variant_ = kLeanTz8;
+  if (lean) variant_ = kLeanTz16RingClamp;
   build_tables();
 }
 
@@ -92,7 +92,7 @@ void Batch::clear_gradient_tables() {
 void Batch::set_variant(int variant, int xcd_remap) {
   if (variant != kFusedTz8 && variant != kFusedTz16 && variant != kTwoStepTz8 &&
       variant != kTwoStepTz16 && !step_is_lean(variant))
-    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8-13 or 18-26");
+    throw ValueError("batched runs use step variants 0, 1, 6, 7, 8-13 or 18-28");
   if (step_is_lean(variant))
     for (auto &m : models_)
       if (!m->lean_ok())
@@ -304,7 +304,7 @@ void Batch::enqueue_gradient_loop(int i_shot, bool forward) {
 
 void Batch::gradients() {
   if (!batch_gradient_variant_ok(variant_))
-    throw ValueError("batched gradients need step variant 0, 1, 8-13 or 18-26");
+    throw ValueError("batched gradients need step variant 0, 1, 8-13 or 18-28");
   si_ = models_[0]->config().snapshot_interval;
   for (auto &m : models_)
     if (m->config().snapshot_interval != si_)
@@ -362,7 +362,7 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
       kFusedTz8, kFusedTz16, kTwoStepTz8, kTwoStepTz16, kLeanTz16,
       kLeanTz8,  kLeanTz12,  kLeanTz20,   kLeanTz24,    kLeanTz32,
       kLeanTz4,  kLeanTz8W6, kLeanTz12W5, kLeanTz8Sep, kLeanTz4Sep, kLeanTz8Ring,
-      kLeanTz16Ring};
+      kLeanTz16Ring, kLeanTz8RingClamp, kLeanTz16RingClamp};
   const int keep_remap = xcd_remap_;
   double best_ms = std::numeric_limits<double>::infinity();
   int best_v = variant_;

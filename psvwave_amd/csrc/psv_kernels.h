@@ -190,7 +190,8 @@ __host__ __device__ inline int taper_count(int x, int z, int nx, int nz, int np)
 // 64 x 8 and 64 x 12 lean kernels held to 6 and 5 waves per SIMD, 21 the
 // 64 x 8 lean kernel with the velocity halo columns in the row transfers,
 // 22 / 23 the 64 x 8 / 64 x 4 lean kernels with separate velocity and
-// stress LDS stages.
+// stress LDS stages, 24-26 lean kernels with ring operands by DMA, 27 / 28
+// variants 24 / 26 with their loads capped to the grid.
 enum StepVariant {
   kFusedTz8 = 0,
   kFusedTz16 = 1,
@@ -219,7 +220,9 @@ enum StepVariant {
   kLeanTz8Ring = 24,  // lean 64 x 8, halo columns and ring operands by DMA
   kLeanTz8RingOnly = 25,  // lean 64 x 8, ring operands by DMA
   kLeanTz16Ring = 26,     // lean 64 x 16, ring operands by DMA (4 waves per SIMD)
-  kNumVariants = 27
+  kLeanTz8RingClamp = 27,   // 24 with loads capped to the grid
+  kLeanTz16RingClamp = 28,  // 26 with loads capped to the grid
+  kNumVariants = 29
 };
 constexpr int kMaxTileHeight = 32;
 

@@ -495,6 +495,32 @@ __device__ __forceinline__ void dma_row70(const double *row_base, lds_dbl *lds_r
                                      0, 0);
 }
 
+// dma_row and dma_row70 with every lane's dword index capped at dlim: the
+// lanes past the grid's last column re-read that column's dword (a line the
+// row fetches anyway) instead of the zero padding beyond it.  row_base is
+// the global address of the row's first staged column.
+__device__ __forceinline__ void dma_row_capped(const double *row_base, lds_dbl *lds_row,
+                                               unsigned dlim) {
+  const unsigned lane = threadIdx.x & 63;
+  const char *g = (const char *)row_base;
+  __builtin_amdgcn_global_load_lds((const void *)(g + min(lane, dlim) * 4u),
+                                   (lds_void *)lds_row, 4, 0, 0);
+  __builtin_amdgcn_global_load_lds((const void *)(g + min(lane + 64u, dlim) * 4u),
+                                   (lds_void *)(lds_row + 32), 4, 0, 0);
+}
+__device__ __forceinline__ void dma_row70_capped(const double *row_base, lds_dbl *lds_row,
+                                                 unsigned dlim) {
+  const unsigned lane = threadIdx.x & 63;
+  const char *g = (const char *)row_base;
+  __builtin_amdgcn_global_load_lds((const void *)(g + min(lane, dlim) * 4u),
+                                   (lds_void *)lds_row, 4, 0, 0);
+  __builtin_amdgcn_global_load_lds((const void *)(g + min(lane + 64u, dlim) * 4u),
+                                   (lds_void *)(lds_row + 32), 4, 0, 0);
+  if (lane < 12)
+    __builtin_amdgcn_global_load_lds((const void *)(g + min(lane + 128u, dlim) * 4u),
+                                     (lds_void *)(lds_row + 64), 4, 0, 0);
+}
+
 __device__ __forceinline__ double ldb(const double *base, unsigned off) {
   return *(const double *)((const char *)base + off);
 }
@@ -549,8 +575,17 @@ __device__ __forceinline__ unsigned lean_off(int r, int c, unsigned rowb) {
 // also go straight into an LDS stage by DMA instead of waiting in
 // registers from the first load to the stress pass; with HALO_DMA this
 // frees enough registers for six waves per SIMD.
-template <int TZ, bool CORR, bool HALO_DMA = false, bool SEP = false, bool RDMA = false>
+// CLAMP (with RDMA): loads of rows below the grid's last row or above its
+// first, and of columns right of its last, read that last (first) row or
+// column instead of the zero padding.  No grid point's stencil reads a value
+// outside the grid (interior points read at most two rows or columns away,
+// the two-point rim is copied), so those values are never used; the tiles
+// on the grid's right and bottom edges then fetch only the lines that hold
+// grid points.  Stores are unchanged.
+template <int TZ, bool CORR, bool HALO_DMA = false, bool SEP = false, bool RDMA = false,
+          bool CLAMP = false>
 __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, const int it) {
+  static_assert(!CLAMP || RDMA, "clamped loads are written for the ring-DMA layout");
   using L = Lean<TZ>;
   constexpr int R = L::R;
   // One LDS buffer: velocities while the stresses are computed, then the
@@ -594,6 +629,16 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   const double *vxi = a.vx_i + tb, *vzi = a.vz_i + tb;
   const double *txxi = a.txx_i + tb, *tzzi = a.tzz_i + tb, *txzi = a.txz_i + tb;
   const double *lmp = a.lm + tb, *mup = a.mu + tb, *bp = a.b + tb;
+  // CLAMP: row and column of a load, capped to the grid (tile-relative
+  // bounds: rows -z0 .. nz-1-z0, columns up to nx-1-x0); lc = the lane's
+  // column so capped
+  const int rlo = -z0, rhi = a.nz - 1 - z0, cmax = a.nx - 1 - x0;
+  auto rc = [&](int r) { return CLAMP ? min(max(r, rlo), rhi) : r; };
+  auto cc = [&](int c) { return CLAMP ? min(c, cmax) : c; };
+  const int lc = cc(lane);
+  // the last dword a row transfer may read: column cmax of a 64-column row
+  // (columns 0..63) or of a 70-column row (columns -3..66)
+  const unsigned dl64 = 2u * (unsigned)cmax + 1u, dl70 = 2u * (unsigned)cmax + 7u;
 
   // ---- phase A: every global load before the first wait ----
   {
@@ -623,7 +668,16 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   for (int k = 0; k < L::KV; ++k) {
     const int r = w + 4 * (k - 1);
     if (r >= -3 && r <= TZ + 2) {
-      if (HALO_DMA) {
+      if (CLAMP) {
+        const long long ro = (long long)(rc(r) + 3) * a.pitch;  // row rc(r), column -8
+        if (HALO_DMA) {
+          dma_row70_capped(vxi + ro + 5, d_vx + L::V(r, -3), dl70);
+          dma_row70_capped(vzi + ro + 5, d_vz + L::V(r, -3), dl70);
+        } else {
+          dma_row_capped(vxi + ro + 8, d_vx + L::V(r, 0), dl64);
+          dma_row_capped(vzi + ro + 8, d_vz + L::V(r, 0), dl64);
+        }
+      } else if (HALO_DMA) {
         const long long ro = tb + (long long)(r + 3) * a.pitch + 5;  // column -3
         dma_row70(a.vx_i + ro, d_vx + L::V(r, -3));
         dma_row70(a.vz_i + ro, d_vz + L::V(r, -3));
@@ -644,7 +698,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     if (!HALO_DMA && k * 256 < L::NSIDE && t < L::NSIDE) {
       int r = t / 6 - 3, cs = t - (t / 6) * 6;
       int c = cs < 3 ? cs - 3 : 61 + cs;
-      unsigned o = off(r, c);
+      unsigned o = off(rc(r), cc(c));
       vxs[k] = ldb(vxi, o);
       vzs[k] = ldb(vzi, o);
       sidx[k] = L::V(r, c);
@@ -653,7 +707,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
   double oxx[R], ozz[R], oxz[R], olm[R], omu[R], ob[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    unsigned o = off(w + 4 * j, lane);
+    unsigned o = off(rc(w + 4 * j), lc);
     oxx[j] = ldb(txxi, o);
     ozz[j] = ldb(tzzi, o);
     oxz[j] = ldb(txzi, o);
@@ -695,14 +749,24 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
     // ring row fields into s_gr[w][field], ring-column items (wave 3, two
     // lanes per value: lane 2k + h moves dword h of item 32 m + k) into
     // s_gc[field]
-    const unsigned o = off(rr, lane);
     lds_dbl *const g = d_gr + w * 4 * 64;
-    if (w != 3) dma_row(txzi, o, g);
-    if (w != 0) {
-      dma_row(tzzi, o, g + 64);
-      dma_row(lmp, o, g + 128);
+    if (CLAMP) {
+      const long long ro = (long long)(rc(rr) + 3) * a.pitch + 8;  // row rc(rr), column 0
+      if (w != 3) dma_row_capped(txzi + ro, g, dl64);
+      if (w != 0) {
+        dma_row_capped(tzzi + ro, g + 64, dl64);
+        dma_row_capped(lmp + ro, g + 128, dl64);
+      }
+      dma_row_capped(mup + ro, g + 192, dl64);
+    } else {
+      const unsigned o = off(rr, lane);
+      if (w != 3) dma_row(txzi, o, g);
+      if (w != 0) {
+        dma_row(tzzi, o, g + 64);
+        dma_row(lmp, o, g + 128);
+      }
+      dma_row(mup, o, g + 192);
     }
-    dma_row(mup, o, g + 192);
     if (w == 3) {
 #pragma unroll
       for (int m = 0; m < NCQ / 32; ++m) {
@@ -710,7 +774,7 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
         if (i < L::NCI) {
           const int q = i / TZ, r = i - q * TZ;
           const int c = q < 2 ? q - 2 : 62 + q;
-          const unsigned oc = off(r, c) + (lane & 1) * 4u;
+          const unsigned oc = off(rc(r), cc(c)) + (lane & 1) * 4u;
           const double *const src[4] = {txxi, txzi, lmp, mup};
 #pragma unroll
           for (int f = 0; f < 4; ++f)
@@ -1006,12 +1070,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
 // CORR: instantiation for adjoint snapshot steps (kernel correlation; one
 // wave per SIMD fewer, for the correlation operands in flight).
 template <int TZ, int WPE, bool CORR, bool HALO_DMA = false, bool SEP = false,
-          bool RDMA = false>
+          bool RDMA = false, bool CLAMP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 step_lean(StepArgs a) {
   const int tile = block_tile(a);
   if (tile >= a.ntiles) return;
-  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA>(a, tile, a.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA, CLAMP>(a, tile, a.it);
 }
 
 // ---------------------------------------------------------------------------
@@ -2195,7 +2259,7 @@ __global__ void __launch_bounds__(NT) batch_two(BatchLaunch b) {
 }
 
 template <int TZ, int WPE, bool CORR = false, bool HALO_DMA = false, bool SEP = false,
-          bool RDMA = false>
+          bool RDMA = false, bool CLAMP = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE)))
 batch_lean(BatchLaunch b) {
   const int g = batch_tile(b);
@@ -2205,7 +2269,7 @@ batch_lean(BatchLaunch b) {
   const int shot = all ? g % b.n_shots : b.shot;
   StepArgs a = b.descs[((long long)mt.x * b.n_shots + shot) * 2 + b.parity];
   if (b.snapshots) batch_snapshot_slot(a, b);
-  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA>(a, mt.y, b.it);
+  lean_tile<TZ, CORR, HALO_DMA, SEP, RDMA, CLAMP>(a, mt.y, b.it);
 }
 
 // Persistent batched forward: one cooperative launch advances steps
@@ -2555,6 +2619,7 @@ int step_tile_height(int variant) {
     case kTwoStepTz16:
     case kLeanTz16:
     case kLeanTz16Ring:
+    case kLeanTz16RingClamp:
       return 16;
     case kLeanTz12:
       return 12;
@@ -2575,7 +2640,7 @@ int step_tile_height(int variant) {
 }
 bool step_is_lean(int variant) {
   return (variant >= kLeanTz16 && variant <= kLeanTz32) ||
-         (variant >= kLeanTz4 && variant <= kLeanTz16Ring);
+         (variant >= kLeanTz4 && variant <= kLeanTz16RingClamp);
 }
 bool fast_division_ok(double d) {
   return d >= 0x1p-40 && d <= 0x1p40;  // false for NaN
@@ -2689,6 +2754,22 @@ hipError_t launch_batch_step(int variant, bool two, const BatchLaunch &b,
         else
           hipExtLaunchKernelGGL((batch_lean<16, 4, false, false, false, true>), g, t, 0, s, e0,
                                 e1, 0, b);
+        break;
+      case kLeanTz8RingClamp:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<8, 5, true, true, false, true, true>), g, t, 0, s,
+                                e0, e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<8, 6, false, true, false, true, true>), g, t, 0, s,
+                                e0, e1, 0, b);
+        break;
+      case kLeanTz16RingClamp:
+        if (corr)
+          hipExtLaunchKernelGGL((batch_lean<16, 3, true, false, false, true, true>), g, t, 0, s,
+                                e0, e1, 0, b);
+        else
+          hipExtLaunchKernelGGL((batch_lean<16, 4, false, false, false, true, true>), g, t, 0,
+                                s, e0, e1, 0, b);
         break;
       default:
         return hipErrorInvalidValue;
@@ -2859,6 +2940,22 @@ hipError_t launch_step(int variant, const StepArgs &a, hipStream_t s,
       else
         hipExtLaunchKernelGGL((step_lean<16, 4, false, false, false, true>), g, b, 0, s, e0, e1,
                               0, a);
+      break;
+    case kLeanTz8RingClamp:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<8, 5, true, true, false, true, true>), g, b, 0, s, e0,
+                              e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<8, 6, false, true, false, true, true>), g, b, 0, s, e0,
+                              e1, 0, a);
+      break;
+    case kLeanTz16RingClamp:
+      if (corr)
+        hipExtLaunchKernelGGL((step_lean<16, 3, true, false, false, true, true>), g, b, 0, s,
+                              e0, e1, 0, a);
+      else
+        hipExtLaunchKernelGGL((step_lean<16, 4, false, false, false, true, true>), g, b, 0, s,
+                              e0, e1, 0, a);
       break;
     case kLeanPipe2Tz8:
     case kLeanPipe3Tz8:

@@ -37,7 +37,7 @@ std::string zero_pad(int num, int pad) {
 
 Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
   lean_ok_ = lean_eligible();
-  if (lean_ok_) variant_ = kLeanTz8Ring;
+  if (lean_ok_) variant_ = kLeanTz8RingClamp;
   allocate();
   // homogeneous background then Lame fields (src/fdModel.cpp:248-261)
   hip_check(launch_fill(vp_, ngrid_, cfg_.scalar_vp, stream_), "fill vp");

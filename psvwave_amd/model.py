@@ -391,10 +391,11 @@ class fdModel:
         lean two steps per launch; 15-17 lean strips of 2-4
         tiles per block; 18 lean 64x4; 19, 20 lean 64x8 / 64x12 at 6 / 5
         waves per SIMD; 21 lean 64x8 with halo columns by DMA; 22, 23 lean
-        64x8 / 64x4 with separate stress stages; 24 (the default) lean 64x8
+        64x8 / 64x4 with separate stress stages; 24 lean 64x8
         with halo columns and ring operands by DMA (6 waves per SIMD); 25
         the same without the halo columns; 26 lean 64x16 with ring operands
-        by DMA (4 waves per SIMD)."""
+        by DMA (4 waves per SIMD); 27 (the default), 28 variants 24, 26 with
+        their loads capped to the grid (edge tiles read no padding)."""
         _lib.check(self._lib.psv_set_step_variant(self._h, int(variant),
                                                   int(bool(xcd_remap))))
 

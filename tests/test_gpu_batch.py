@@ -38,7 +38,8 @@ def _fields(nx, nz, seed):
                                            (11, False), (12, False), (13, False),
                                            (18, False), (19, False), (20, False),
                                            (21, False), (22, False), (23, False),
-                                           (24, False), (25, False), (26, False)])
+                                           (24, False), (25, False), (26, False),
+                                           (27, False), (28, False)])
 def test_batch_members_equal_single_models(oracle_lib, variant, graph, persistent=False):
     import psvwave_amd
     cfgs = _members()
@@ -78,7 +79,7 @@ def test_batch_members_equal_single_models(oracle_lib, variant, graph, persisten
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("variant", [9, 10, 18, 22, 24])
+@pytest.mark.parametrize("variant", [9, 10, 18, 22, 24, 27, 28])
 def test_forward_all_lockstep_equals_shot_by_shot(variant):
     """forward_all advances both shots in the same launches (second shot in a
     second state buffer set); the seismograms equal forward(0), forward(1)
@@ -123,7 +124,7 @@ def test_batch_autotune_and_validation():
     cfgs = _members()
     batch = psvwave_amd.Batch(cfgs[:2])
     best, times = batch.autotune(steps=6)
-    tuned = {0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23, 24, 26}
+    tuned = {0, 1, 6, 7, 8, 9, 10, 11, 12, 13, 18, 19, 20, 22, 23, 24, 26, 27, 28}
     assert best in tuned
     assert set(times) == tuned
     assert all(t > 0 for t in times.values())
@@ -134,7 +135,7 @@ def test_batch_autotune_and_validation():
         batch.set_variant(2)
 
 
-@pytest.mark.parametrize("variant", [9, 0, 18, 22, 23, 8, 13, 24, 26])
+@pytest.mark.parametrize("variant", [9, 0, 18, 22, 23, 8, 13, 24, 26, 27, 28])
 def test_batch_gradients_equal_single_models(oracle_lib, variant):
     """Batched FWI gradients (forward with snapshots, adjoint with kernel
     correlation, shot by shot over all members) equal each member's own

@@ -68,7 +68,7 @@ def test_sources_and_receivers_on_tile_boundaries(oracle_lib):
                           moment_angles=[30, 60, 90])
     cfg["receivers"].update(ix_receivers=[52, 51, 52, 52, 63, 0, 89],
                             iz_receivers=[4, 4, 3, 4, 11, 0, 49])
-    for variant in range(27):
+    for variant in range(29):
         p, o = _run_both(oracle_lib, cfg, variant)
         _assert_same(p, o)
 
@@ -82,7 +82,8 @@ def test_coincident_sources_and_receivers_keep_reference_order(oracle_lib):
                           n_shots=2)
     cfg["receivers"].update(nr=5, ix_receivers=[40, 40, 40, 3, 40],
                             iz_receivers=[30, 30, 30, 4, 31])
-    for variant in (0, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26):
+    for variant in (0, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25,
+                    26, 27, 28):
         p, o = _run_both(oracle_lib, cfg, variant)
         _assert_same(p, o)
 
@@ -100,7 +101,8 @@ def test_odd_grid_sizes(oracle_lib, nx_inner, nz_inner, np_b):
                             iz_receivers=[nz_inner - 1, 0])
     if np_b == 0:  # a source needs one point of margin around it
         cfg["sources"]["ix_sources"] = [max(1, nx_inner // 2)]
-    for variant in (0, 2, 3, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26):
+    for variant in (0, 2, 3, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23,
+                    24, 25, 26, 27, 28):
         p, o = _run_both(oracle_lib, cfg, variant)
         _assert_same(p, o)
 
@@ -119,7 +121,8 @@ def test_wavefront_tail_values_match_bitwise(oracle_lib):
     far = np.abs(o["syn"][0][0, 1])  # the far receiver's trace
     assert far[far > 0].min() < 2.0 ** -960
     _assert_same(p, o)
-    for variant in (0, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26):
+    for variant in (0, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26,
+                    27, 28):
         _assert_same(_run_product_only(cfg, variant, True), o)
 
 
@@ -130,7 +133,7 @@ def test_variants_and_xcd_mapping_agree_bitwise(oracle_lib):
     cfg["receivers"]["ix_receivers"] = [5, 17, 130, 151, 152, 275, 288]
     p, o = _run_both(oracle_lib, cfg, 0, True)
     _assert_same(p, o)
-    for variant in range(27):
+    for variant in range(29):
         for remap in (True, False):
             _assert_same(_run_product_only(cfg, variant, remap), o)
 

@@ -114,7 +114,7 @@ def _small_models(cfg):
     return true, start
 
 
-@pytest.mark.parametrize("variant", list(range(27)))
+@pytest.mark.parametrize("variant", list(range(29)))
 def test_small_chain_matches_oracle(oracle_lib, variant):
     cfg = small_config()
     true, start = _small_models(cfg)
