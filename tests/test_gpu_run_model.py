@@ -59,7 +59,7 @@ def _outputs(m, adjoint):
     return out
 
 
-@pytest.mark.parametrize("variant", [9, 0, 2, 6, 14])
+@pytest.mark.parametrize("variant", [27, 9, 0, 2, 6, 14])
 @pytest.mark.parametrize("adjoint", [True, False])
 @pytest.mark.parametrize("n_shots", [4, 2])
 def test_two_lane_run_model_is_bitwise_sequential(variant, adjoint, n_shots):
@@ -73,6 +73,24 @@ def test_two_lane_run_model_is_bitwise_sequential(variant, adjoint, n_shots):
     a.run_model(False, adjoint)
     for k, (x, y) in enumerate(zip(_outputs(a, adjoint), _outputs(b, adjoint))):
         assert np.array_equal(x, y), (variant, adjoint, "again", k)
+
+
+@pytest.mark.parametrize("variant", [27, 0])
+def test_deferred_terms_ignore_earlier_calls(variant):
+    """The deferred adjoint writes its per-slot kernel terms without reading
+    its buffer, which is not cleared between calls: run_model at new
+    parameters, after a run that left other terms in the buffer, equals the
+    sequential calls at the new parameters."""
+    cfg = _config(4)
+    a, b = _prepared(cfg, variant), _prepared(cfg, variant)
+    a.run_model(False, True)
+    new = _fields(a.nx, a.nz, 3)
+    a.set_parameter_fields(*new)
+    b.set_parameter_fields(*new)
+    a.run_model(False, True)
+    _sequential(b, True)
+    for k, (x, y) in enumerate(zip(_outputs(a, True), _outputs(b, True))):
+        assert np.array_equal(x, y), (variant, k)
 
 
 def test_run_model_timing_and_fallbacks():
