@@ -265,6 +265,13 @@ int psv_l2_misfit_noalloc(const double *syn_ux, const double *syn_uz,
  * byte counters; 1: hipMemcpyAsync (the runtime's own copy), the practical
  * streaming ceiling. */
 int psv_calibrate_copy(int64_t bytes, int iters, int runtime_copy, double *ms);
+/* No counterpart in the reference (its loops run on host threads).  HIP
+ * streams the library holds open in this process and the runtime's
+ * hardware-queue limit (GPU_MAX_HW_QUEUES, 4 by default): a model's two
+ * lanes overlap only while they have a queue each, so more live streams
+ * than queues (the default stream counts too) may run the lanes one after
+ * the other -- same results, no overlap. */
+int psv_stream_pressure(int *live_streams, int *hw_queues);
 
 /* ---- Batched Monte-Carlo forward runs (no reference counterpart: the
  * reference runs one fdModel at a time, README.md:16-17 states the

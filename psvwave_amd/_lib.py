@@ -115,6 +115,7 @@ _SIGNATURES = {
     "psv_reset_step_timing": [_P],
     "psv_device_bytes": [_P, C.POINTER(C.c_int64)],
     "psv_calibrate_copy": [C.c_int64, _i, _i, _D],
+    "psv_stream_pressure": [_I, _I],
     "psv_l2_misfit_noalloc": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                               C.c_double, C.c_void_p, C.c_void_p],
     "psv_batch_create": [_i, C.POINTER(Params), C.POINTER(_P)],

@@ -29,8 +29,7 @@ Batch::Batch(const std::vector<Config> &cfgs) {
     if (c.nt != nt_ || c.n_shots != n_shots_)
       throw ValueError("all models of a batch must share nt and n_shots");
   for (const Config &c : cfgs) models_.emplace_back(new Model(c));
-  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
-            "hipStreamCreate");
+  create_stream(&stream_);
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
   for (auto &m : models_) points_ += m->grid_points();
@@ -52,7 +51,7 @@ Batch::~Batch() {
   clear_gradient_tables();
   for (auto e : ev_)
     if (e) (void)hipEventDestroy(e);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  destroy_stream(stream_);
 }
 
 Model &Batch::model(int m) {

@@ -599,6 +599,15 @@ int psv_device_bytes(const psv_model *m, int64_t *bytes) {
   });
 }
 
+int psv_stream_pressure(int *live_streams, int *hw_queues) {
+  return guard([&] {
+    need(live_streams, "live_streams");
+    need(hw_queues, "hw_queues");
+    *live_streams = psv::live_streams();
+    *hw_queues = psv::hw_queue_limit();
+  });
+}
+
 int psv_calibrate_copy(int64_t bytes, int iters, int runtime_copy, double *ms) {
   return guard([&] {
     need(ms, "ms");

@@ -20,6 +20,15 @@ struct RuntimeError : public std::runtime_error {
   explicit RuntimeError(const std::string &m) : std::runtime_error(m) {}
 };
 
+// HIP streams the library holds open in this process (model, lane and
+// batch streams).  Two time loops overlap only on two hardware queues: the
+// HIP runtime creates at most GPU_MAX_HW_QUEUES (4 by default) per device
+// and shares them among the streams beyond that.
+void create_stream(hipStream_t *s);  // hipStreamNonBlocking; throws on failure
+void destroy_stream(hipStream_t s);
+int live_streams();
+int hw_queue_limit();  // GPU_MAX_HW_QUEUES, or 4 when unset or invalid
+
 class Model {
  public:
   explicit Model(const Config &c);

@@ -7,7 +7,9 @@
 #include "psv_model.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -33,7 +35,30 @@ std::string zero_pad(int num, int pad) {
   return ss.str();
 }
 
+std::atomic<int> g_live_streams{0};
+
 }  // namespace
+
+void create_stream(hipStream_t *s) {
+  hip_check(hipStreamCreateWithFlags(s, hipStreamNonBlocking), "hipStreamCreate");
+  ++g_live_streams;
+}
+
+void destroy_stream(hipStream_t s) {
+  if (!s) return;
+  (void)hipStreamDestroy(s);
+  --g_live_streams;
+}
+
+int live_streams() { return g_live_streams.load(); }
+
+int hw_queue_limit() {
+  const char *v = std::getenv("GPU_MAX_HW_QUEUES");
+  if (!v || !*v) return 4;
+  char *end = nullptr;
+  const long n = std::strtol(v, &end, 10);
+  return (*end == '\0' && n > 0 && n < 1024) ? (int)n : 4;
+}
 
 Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
   lean_ok_ = lean_eligible();
@@ -96,10 +121,10 @@ Model::~Model() {
   for (auto e : ckpt_events_) (void)hipEventDestroy(e);
   for (auto e : job_events_) (void)hipEventDestroy(e);
   if (ev_join_) (void)hipEventDestroy(ev_join_);
-  if (lane_stream_) (void)hipStreamDestroy(lane_stream_);
+  destroy_stream(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  destroy_stream(stream_);
 }
 
 double *Model::dalloc(size_t n) {
@@ -119,8 +144,7 @@ void Model::sync() const {
 }
 
 void Model::allocate() {
-  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
-            "hipStreamCreate");
+  create_stream(&stream_);
   ngrid_ = (long long)d_.nx * d_.nz;
   nrec_ = (long long)cfg_.n_shots * cfg_.nr * cfg_.nt;
   // Padded stencil layout, large enough for any tile height in use.
@@ -641,9 +665,7 @@ bool Model::async_path(bool verbose, bool dump) const {
 // Lane Y: its stream and state buffers.  Allocation zero-fills on the model
 // stream, so the first use waits for it.
 void Model::ensure_lane() {
-  if (!lane_stream_)
-    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
-              "hipStreamCreate");
+  if (!lane_stream_) create_stream(&lane_stream_);
   if (!ev_join_) hip_check(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming),
                            "hipEventCreate");
   if ((int)fwd_job_.size() < cfg_.n_shots) fwd_job_.assign(cfg_.n_shots, -1);
@@ -791,9 +813,7 @@ void Model::enqueue_checkpointed_adjoint(
     }
     seg_k_ = k;
   }
-  if (!lane_stream_)
-    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
-              "hipStreamCreate");
+  if (!lane_stream_) create_stream(&lane_stream_);
   while (ckpt_events_.size() < 6) {
     hipEvent_t e;
     hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
@@ -1293,9 +1313,7 @@ bool Model::ensure_defer_buffer() {
 void Model::run_model_two_lanes(bool adj) {
   const int ns = cfg_.n_shots, nt = cfg_.nt;
   ensure_snapshots();
-  if (!lane_stream_)
-    hip_check(hipStreamCreateWithFlags(&lane_stream_, hipStreamNonBlocking),
-              "hipStreamCreate");
+  if (!lane_stream_) create_stream(&lane_stream_);
   if (!lane_state_[0][0])
     for (int p = 0; p < 2; ++p)
       for (int f = 0; f < 5; ++f) lane_state_[p][f] = dalloc(npad_);

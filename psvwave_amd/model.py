@@ -6,10 +6,35 @@ the pybind11 class of psvWave (src/psvWave.cpp:465-655, fdModelExtended at
 are copies in the reference layout (grids (nx, nz) with iz fastest).
 """
 import ctypes as C
+import warnings
 
 import numpy as np
 
 from . import _lib
+
+# set once the queue-sharing warning has been given in this process
+_queue_warning_issued = False
+
+
+def _check_queue_sharing(lib):
+    """Warn (once per process) when the library holds more HIP streams, with
+    the default stream, than the runtime has hardware queues: a model's two
+    lanes may then share a queue and run one after the other.  Results are
+    unchanged; only the overlap of the two time loops is lost."""
+    global _queue_warning_issued
+    if _queue_warning_issued:
+        return
+    live, queues = C.c_int(), C.c_int()
+    _lib.check(lib.psv_stream_pressure(C.byref(live), C.byref(queues)))
+    if live.value + 1 > queues.value:
+        _queue_warning_issued = True
+        warnings.warn(
+            f"psvwave_amd: {live.value} HIP streams of this library are live, "
+            f"with the default stream more than the {queues.value} hardware "
+            "queues of GPU_MAX_HW_QUEUES; a model's two lanes may share a queue "
+            "and lose their overlap (results are unchanged).  Release models "
+            "no longer used or raise GPU_MAX_HW_QUEUES.", RuntimeWarning,
+            stacklevel=3)
 
 
 def _f64(a, shape, name):
@@ -172,6 +197,7 @@ class fdModel:
                                          int(bool(store_fields)),
                                          int(bool(verbose)),
                                          int(bool(output_wavefields))))
+        _check_queue_sharing(self._lib)
 
     def adjoint_simulate(self, i_shot, verbose=False, omp_threads_override=0):
         """Adjoint simulation of shot `i_shot`, correlating with the stored
@@ -179,6 +205,7 @@ class fdModel:
         Returns once the time loop is enqueued (see set_async)."""
         _lib.check(self._lib.psv_adjoint(self._h, int(i_shot),
                                          int(bool(verbose))))
+        _check_queue_sharing(self._lib)
 
     def calculate_l2_misfit(self):
         """Calculate the L2 misfit of synthetic w.r.t. observed data."""
@@ -204,6 +231,7 @@ class fdModel:
         self._generation += 1
         _lib.check(self._lib.psv_run_model(self._h, int(bool(verbose)),
                                            int(bool(simulate_adjoint))))
+        _check_queue_sharing(self._lib)
         self._refresh_misfit()
 
     def _refresh_misfit(self):

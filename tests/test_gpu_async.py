@@ -121,3 +121,28 @@ def test_synchronous_variants_and_copies():
         m.forward_simulate(s)
     c = m.copy()
     _assert_same(c.get_synthetic_data(), m.get_synthetic_data(), "copy")
+
+
+def test_warning_when_lanes_may_share_a_queue():
+    """With more live library streams (the default stream included) than
+    the runtime's hardware queues, a model's two lanes may share a queue:
+    the library warns once per process, and results stay those of the
+    synchronous calls."""
+    import gc
+    import psvwave_amd
+    import psvwave_amd.model as pm
+    gc.collect()
+    cfg = _shots(2)
+    want = _gradient(_model(cfg, False))
+    pm._queue_warning_issued = False
+    models = []
+    with pytest.warns(RuntimeWarning, match="hardware"):
+        while not pm._queue_warning_issued:
+            assert len(models) < 64
+            m = psvwave_amd.fdModel(cfg)
+            m.forward_simulate(0, store_fields=False)  # opens the model's second lane
+            models.append(m)
+    got = _gradient(_model(cfg, True))
+    _assert_same(got, want, "shared queues")
+    del models
+    gc.collect()

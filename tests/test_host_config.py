@@ -33,6 +33,30 @@ def test_library_exports_every_declared_symbol():
     assert names <= set(_lib._SIGNATURES), names - set(_lib._SIGNATURES)
 
 
+def test_stream_pressure_reads_the_queue_limit(monkeypatch):
+    """psv_stream_pressure makes no HIP call: with no model alive it reports
+    no library streams, and the hardware-queue limit from GPU_MAX_HW_QUEUES
+    (4 when unset or not a positive number)."""
+    import ctypes as C
+    from psvwave_amd import _lib
+    lib = _lib.load()
+    live, queues = C.c_int(-1), C.c_int(-1)
+
+    def read():
+        _lib.check(lib.psv_stream_pressure(C.byref(live), C.byref(queues)))
+        return live.value, queues.value
+
+    monkeypatch.delenv("GPU_MAX_HW_QUEUES", raising=False)
+    assert read() == (0, 4)
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "7")
+    assert read() == (0, 7)
+    for bad in ("0", "-3", "x", "5x", ""):
+        monkeypatch.setenv("GPU_MAX_HW_QUEUES", bad)
+        assert read() == (0, 4), bad
+    with pytest.raises(ValueError):
+        _lib.check(lib.psv_stream_pressure(None, C.byref(queues)))
+
+
 def test_default_ini_parses_like_the_explicit_values():
     """Restates tests/test_constructor_comparison.cpp at the configuration
     level: the INI file and the explicit values give the same model."""
