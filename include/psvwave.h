@@ -299,8 +299,8 @@ int psv_batch_forward(psv_batch *b, int i_shot);
  * step variant both shots advance together, one launch per time step over
  * every (tile, shot) of every model, the second shot in each model's second
  * state buffer set (10 padded grids more per model; shot by shot when the
- * device has no room for them, in graph mode and with the persistent
- * loop). */
+ * device has no room for them and with the persistent loop).  In graph mode
+ * that whole loop is captured once as one HIP graph and replayed. */
 int psv_batch_forward_all(psv_batch *b);
 /* *on = 1 when psv_batch_forward_all advances the shots together with the
  * current variant and modes (allocating the second state sets if needed),

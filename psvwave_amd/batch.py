@@ -81,7 +81,8 @@ class Batch:
         forward(0), forward(1), ... give.  With two shots and a lean step
         variant both shots advance in the same launches (each tile's two
         shots side by side, sharing their material reads), the second shot
-        in a second state buffer set per member."""
+        in a second state buffer set per member; in graph mode the whole
+        loop is captured once as one HIP graph and replayed."""
         _lib.check(self._lib.psv_batch_forward_all(self._h))
 
     def lockstep(self):

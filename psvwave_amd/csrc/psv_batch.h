@@ -39,11 +39,12 @@ class Batch {
   // Forward simulation of shot i_shot for every member (no snapshots).
   void forward(int i_shot);
   // Forward simulation of every shot of every member.  With two shots and
-  // a lean step variant (no graph capture, no persistent loop) both shots
-  // advance together: one launch per time step over every (tile, shot) of
-  // every member, the second shot in each member's second state buffer
-  // set, so the two reads of a tile's materials meet in L2.  Otherwise,
-  // or when the device has no room for the second state sets, shot by shot.
+  // a lean step variant (no persistent loop) both shots advance together:
+  // one launch per time step over every (tile, shot) of every member, the
+  // second shot in each member's second state buffer set, so the two reads
+  // of a tile's materials meet in L2; in graph mode the whole loop is one
+  // captured HIP graph.  Otherwise, or when the device has no room for the
+  // second state sets, shot by shot.
   // Seismograms are those of forward(0), forward(1), ... bit for bit.
   void forward_all();
   bool lockstep_available();
@@ -93,6 +94,7 @@ class Batch {
   bool graph_mode_ = false;
   bool persistent_ = false;
   std::vector<hipGraphExec_t> graphs_;  // per shot
+  hipGraphExec_t graph_all_ = nullptr;  // forward_all's loop of both shots
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double loop_ms_ = 0.0;
   // gradient descriptors [model][shot][parity] with snapshot-store bases

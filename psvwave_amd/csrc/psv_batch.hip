@@ -68,6 +68,8 @@ void Batch::clear_graphs() {
   for (auto g : graphs_)
     if (g) (void)hipGraphExecDestroy(g);
   graphs_.clear();
+  if (graph_all_) (void)hipGraphExecDestroy(graph_all_);
+  graph_all_ = nullptr;
 }
 
 void Batch::set_persistent(bool on) {
@@ -234,7 +236,7 @@ void Batch::forward(int i_shot) {
 }
 
 bool Batch::lockstep_available() {
-  if (n_shots_ != 2 || !step_is_lean(variant_) || graph_mode_ || persistent_ ||
+  if (n_shots_ != 2 || !step_is_lean(variant_) || persistent_ ||
       no_lockstep_room_ || (long long)ntiles_ * n_shots_ > std::numeric_limits<int>::max())
     return false;
   if (ldescs_) return true;
@@ -277,7 +279,35 @@ void Batch::forward_all() {
     return;
   }
   hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
-  enqueue_lockstep_loop(nt_);
+  if (graph_mode_) {
+    // the whole loop (state zeroing and nt launches) captured once and
+    // replayed: one host launch per call; the tables it reads were built
+    // by lockstep_available, outside the capture
+    if (!graph_all_) {
+      hip_check(hipStreamSynchronize(stream_), "sync");
+      hipGraph_t graph = nullptr;
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture");
+      try {
+        enqueue_lockstep_loop(nt_);
+      } catch (...) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(stream_, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+      hipGraphExec_t exec = nullptr;
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      hip_check(e, "hipGraphInstantiate");
+      graph_all_ = exec;
+      hip_check(hipEventRecord(ev_[0], stream_), "hipEventRecord");
+    }
+    hip_check(hipGraphLaunch(graph_all_, stream_), "hipGraphLaunch");
+  } else {
+    enqueue_lockstep_loop(nt_);
+  }
   hip_check(hipEventRecord(ev_[1], stream_), "hipEventRecord");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   float ms = 0.f;

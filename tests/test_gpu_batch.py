@@ -103,9 +103,18 @@ def test_forward_all_lockstep_equals_shot_by_shot(variant):
                 assert np.array_equal(a, b), (variant, m, rep)
     batch.set_variant(0)  # fused steps: shot by shot
     assert not batch.lockstep()
+    # graph mode: the loop of both shots captured once, then replayed with
+    # host work and synchronisation between the replays
     batch.set_variant(variant)
     batch.set_graph_mode(True)
-    assert not batch.lockstep()
+    assert batch.lockstep()
+    for rep in range(3):
+        batch.forward_all()
+        for m in range(len(cfgs)):
+            for a, b in zip(batch.get_synthetic_data(m), want[m]):
+                assert np.array_equal(a, b), (variant, m, "graph", rep)
+        info = batch.infos[rep]
+        batch.set_parameter_fields(rep, *_fields(info["nx"], info["nz"], 50 + rep))
 
 
 def test_persistent_batch_loop_equals_single_models(oracle_lib):
