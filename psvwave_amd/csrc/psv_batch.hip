@@ -28,8 +28,16 @@ Batch::Batch(const std::vector<Config> &cfgs) {
   for (const Config &c : cfgs)
     if (c.nt != nt_ || c.n_shots != n_shots_)
       throw ValueError("all models of a batch must share nt and n_shots");
-  for (const Config &c : cfgs) models_.emplace_back(new Model(c));
+  // the members run their own calls (parameters, adjoint sources, misfits)
+  // on the batch's stream: one stream for the whole batch
   create_stream(&stream_);
+  try {
+    for (const Config &c : cfgs) models_.emplace_back(new Model(c, stream_));
+  } catch (...) {
+    models_.clear();
+    destroy_stream(stream_);
+    throw;
+  }
   hip_check(hipEventCreate(&ev_[0]), "hipEventCreate");
   hip_check(hipEventCreate(&ev_[1]), "hipEventCreate");
   for (auto &m : models_) points_ += m->grid_points();
@@ -51,6 +59,7 @@ Batch::~Batch() {
   clear_gradient_tables();
   for (auto e : ev_)
     if (e) (void)hipEventDestroy(e);
+  models_.clear();  // the members run on stream_
   destroy_stream(stream_);
 }
 

@@ -31,7 +31,9 @@ int hw_queue_limit();  // GPU_MAX_HW_QUEUES, or 4 when unset or invalid
 
 class Model {
  public:
-  explicit Model(const Config &c);
+  // shared_stream: run on that stream (a batch's) instead of one of its
+  // own; the caller keeps it alive until the model is destroyed
+  explicit Model(const Config &c, hipStream_t shared_stream = nullptr);
   Model(const Model &other);  // deep copy of device state
   ~Model();
   Model &operator=(const Model &) = delete;
@@ -158,6 +160,7 @@ class Model {
   int pitch_ = 0, rows_ = 0;
   long long origin_ = 0, npad_ = 0, ngrid_ = 0, nrec_ = 0;
   hipStream_t stream_ = nullptr;
+  bool owns_stream_ = true;
   std::vector<void *> allocations_;
   int64_t device_bytes_ = 0;
 

@@ -60,7 +60,11 @@ int hw_queue_limit() {
   return (*end == '\0' && n > 0 && n < 1024) ? (int)n : 4;
 }
 
-Model::Model(const Config &c) : cfg_(c), d_(derive(c)) {
+Model::Model(const Config &c, hipStream_t shared_stream) : cfg_(c), d_(derive(c)) {
+  if (shared_stream) {
+    stream_ = shared_stream;
+    owns_stream_ = false;
+  }
   lean_ok_ = lean_eligible();
   if (lean_ok_) variant_ = kLeanTz8RingClamp;
   allocate();
@@ -124,7 +128,7 @@ Model::~Model() {
   destroy_stream(lane_stream_);
   for (void *p : allocations_) (void)hipFree(p);
   if (staging_) (void)hipFree(staging_);
-  destroy_stream(stream_);
+  if (owns_stream_) destroy_stream(stream_);
 }
 
 double *Model::dalloc(size_t n) {
@@ -144,7 +148,7 @@ void Model::sync() const {
 }
 
 void Model::allocate() {
-  create_stream(&stream_);
+  if (!stream_) create_stream(&stream_);
   ngrid_ = (long long)d_.nx * d_.nz;
   nrec_ = (long long)cfg_.n_shots * cfg_.nr * cfg_.nt;
   // Padded stencil layout, large enough for any tile height in use.
