@@ -146,3 +146,33 @@ def test_warning_when_lanes_may_share_a_queue():
     _assert_same(got, want, "shared queues")
     del models
     gc.collect()
+
+
+def test_stream_count_follows_models_and_batches():
+    """The library's live-stream count rises with each model (its own
+    stream, and its second lane once used) and with each batch (one stream
+    for all members), and falls back when they are released."""
+    import ctypes as C
+    import gc
+    import psvwave_amd
+    from psvwave_amd import _lib
+    lib = _lib.load()
+
+    def live():
+        n, q = C.c_int(), C.c_int()
+        _lib.check(lib.psv_stream_pressure(C.byref(n), C.byref(q)))
+        return n.value
+
+    gc.collect()
+    base = live()
+    cfg = _shots(2)
+    m = psvwave_amd.fdModel(cfg)
+    assert live() == base + 1
+    m.forward_simulate(0, store_fields=False)  # opens the second lane
+    m.synchronize()
+    assert live() == base + 2
+    b = psvwave_amd.Batch([cfg] * 3)
+    assert live() == base + 3
+    del m, b
+    gc.collect()
+    assert live() == base
