@@ -550,11 +550,11 @@ __device__ __forceinline__ void lean_store_snapshot(const StepArgs &a, int x, in
     const int z = z0w + 4 * j;
     if (xin && z < a.nz) {
       const long long ci = (long long)z * a.nx + x;
-      a.snap[0][ci] = vx[j];
-      a.snap[1][ci] = vz[j];
-      a.snap[2][ci] = txx[j];
-      a.snap[3][ci] = tzz[j];
-      a.snap[4][ci] = txz[j];
+      __builtin_nontemporal_store(vx[j], a.snap[0] + ci);
+      __builtin_nontemporal_store(vz[j], a.snap[1] + ci);
+      __builtin_nontemporal_store(txx[j], a.snap[2] + ci);
+      __builtin_nontemporal_store(tzz[j], a.snap[3] + ci);
+      __builtin_nontemporal_store(txz[j], a.snap[4] + ci);
     }
   }
 }
@@ -728,12 +728,12 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
       if (cin[j]) {
         const long long ci = (long long)z * a.nx + xx;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) csv[j][f] = a.snap[f][ci];
+        for (int f = 0; f < 5; ++f) csv[j][f] = __builtin_nontemporal_load(a.snap[f] + ci);
         crk[j] = clk[j] = cmk[j] = 0.0;
         if (!a.kfresh) {
-          crk[j] = a.rho_k[ci];
-          clk[j] = a.lam_k[ci];
-          cmk[j] = a.mu_k[ci];
+          crk[j] = __builtin_nontemporal_load(a.rho_k + ci);
+          clk[j] = __builtin_nontemporal_load(a.lam_k + ci);
+          cmk[j] = __builtin_nontemporal_load(a.mu_k + ci);
         }
       }
     }
@@ -860,9 +860,9 @@ __device__ __forceinline__ void lean_tile(const StepArgs &a, const int tile, con
         correlate_point(a.wcorr, olm[j], omu[j], vx0[j], vz0[j], oxx[j], ozz[j], oxz[j],
                         csv[j][0], csv[j][1], csv[j][2], csv[j][3], csv[j][4], crk[j],
                         clk[j], cmk[j]);
-        a.rho_k[ci] = crk[j];
-        a.lam_k[ci] = clk[j];
-        a.mu_k[ci] = cmk[j];
+        __builtin_nontemporal_store(crk[j], a.rho_k + ci);
+        __builtin_nontemporal_store(clk[j], a.lam_k + ci);
+        __builtin_nontemporal_store(cmk[j], a.mu_k + ci);
       }
   } else if (a.do_snap && !a.adjoint) {
     lean_store_snapshot<R>(a, x, z0 + w, xin, vx0, vz0, oxx, ozz, oxz);
@@ -3004,9 +3004,9 @@ __global__ void replay_correlation_kernel(double *rho_k, double *lam_k, double *
   double r = rho_k[ci], l = lam_k[ci], m = mu_k[ci];
   for (int s = nslots - 1; s >= 0; --s) {
     const double *ds = d + (long long)s * 3 * ngrid + ci;
-    r = r + ds[0];
-    l = l + ds[ngrid];
-    m = m + ds[2 * ngrid];
+    r = r + __builtin_nontemporal_load(ds);
+    l = l + __builtin_nontemporal_load(ds + ngrid);
+    m = m + __builtin_nontemporal_load(ds + 2 * ngrid);
   }
   rho_k[ci] = r;
   lam_k[ci] = l;
