@@ -295,15 +295,15 @@ int psv_batch_get_synthetic_data(const psv_batch *b, int m, double *ux,
 /* Forward simulation of shot i_shot for every model (no snapshots). */
 int psv_batch_forward(psv_batch *b, int i_shot);
 /* Forward simulation of every shot of every model (what psv_batch_forward
- * for shots 0 .. n_shots-1 does, bit for bit).  With two shots and a lean
- * step variant both shots advance together, one launch per time step over
- * every (tile, shot) of every model, the second shot in each model's second
- * state buffer set (10 padded grids more per model; shot by shot when the
- * device has no room for them and with the persistent loop).  In graph mode
- * that whole loop is captured once as one HIP graph and replayed. */
+ * for shots 0 .. n_shots-1 does, bit for bit).  With 2 to 8 shots and a
+ * lean step variant the shots advance together, one launch per time step
+ * over every (tile, shot) of every model, shot s in the model's state buffer
+ * set s (10 padded grids more per model and extra shot; shot by shot when
+ * the device has no room for them and with the persistent loop).  In graph
+ * mode that whole loop is captured once as one HIP graph and replayed. */
 int psv_batch_forward_all(psv_batch *b);
 /* *on = 1 when psv_batch_forward_all advances the shots together with the
- * current variant and modes (allocating the second state sets if needed),
+ * current variant and modes (allocating the extra state sets if needed),
  * else 0. */
 int psv_batch_lockstep(psv_batch *b, int *on);
 /* L2 misfit of every model (n values), as calculate_l2_misfit. */

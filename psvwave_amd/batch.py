@@ -78,16 +78,16 @@ class Batch:
 
     def forward_all(self):
         """Forward simulation of every shot for every member: bitwise what
-        forward(0), forward(1), ... give.  With two shots and a lean step
-        variant both shots advance in the same launches (each tile's two
-        shots side by side, sharing their material reads), the second shot
-        in a second state buffer set per member; in graph mode the whole
-        loop is captured once as one HIP graph and replayed."""
+        forward(0), forward(1), ... give.  With 2 to 8 shots and a lean step
+        variant the shots advance in the same launches (each tile's shots
+        side by side, sharing their material reads), shot s in state buffer
+        set s of each member; in graph mode the whole loop is captured once
+        as one HIP graph and replayed."""
         _lib.check(self._lib.psv_batch_forward_all(self._h))
 
     def lockstep(self):
         """Whether forward_all advances the shots together with the current
-        variant and modes (allocates the second state sets if needed)."""
+        variant and modes (allocates the extra state sets if needed)."""
         on = C.c_int()
         _lib.check(self._lib.psv_batch_lockstep(self._h, C.byref(on)))
         return bool(on.value)
@@ -131,7 +131,7 @@ class Batch:
 
     def autotune(self, steps=20):
         """Time variants 0, 1, 6, 7, 8-13, 18-20, 22-24, 26-28 over `steps` steps,
-        each in the loop forward_all runs with it (both shots in lockstep
+        each in the loop forward_all runs with it (all shots in lockstep
         where that applies), and keep the fastest.  Returns (best variant,
         {variant: ms per time step of one shot})."""
         best = C.c_int()

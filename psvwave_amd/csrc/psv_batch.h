@@ -38,13 +38,13 @@ class Batch {
   void set_persistent(bool on);
   // Forward simulation of shot i_shot for every member (no snapshots).
   void forward(int i_shot);
-  // Forward simulation of every shot of every member.  With two shots and
-  // a lean step variant (no persistent loop) both shots advance together:
-  // one launch per time step over every (tile, shot) of every member, the
-  // second shot in each member's second state buffer set, so the two reads
-  // of a tile's materials meet in L2; in graph mode the whole loop is one
-  // captured HIP graph.  Otherwise, or when the device has no room for the
-  // second state sets, shot by shot.
+  // Forward simulation of every shot of every member.  With 2 to
+  // Model::kMaxStateSets shots and a lean step variant (no persistent loop)
+  // the shots advance together: one launch per time step over every (tile,
+  // shot) of every member, shot s in the member's state buffer set s, so
+  // the reads of a tile's materials meet in L2; in graph mode the whole loop
+  // is one captured HIP graph.  Otherwise, or when the device has no room
+  // for the extra state sets, shot by shot.
   // Seismograms are those of forward(0), forward(1), ... bit for bit.
   void forward_all();
   bool lockstep_available();
@@ -61,7 +61,7 @@ class Batch {
   // Times `steps` batched steps of each candidate variant (ms_per_variant
   // gets kBatchCandidates entries, -1 for a variant the batch cannot use),
   // keeps the fastest; returns its time per step of one shot in ms.  Each
-  // variant is timed in the loop forward_all runs with it: both shots in
+  // variant is timed in the loop forward_all runs with it: all shots in
   // lockstep where that applies, else shot 0 alone.
   double autotune(int steps, int *best, double *ms_per_variant);
   long long grid_points() const { return points_; }
@@ -75,7 +75,7 @@ class Batch {
   // every member's state buffer set `set` zeroed in one launch
   void ensure_zero_table(int set);
   void enqueue_zero_states(int set);
-  double **zero_ptrs_[2] = {nullptr, nullptr};
+  double **zero_ptrs_[Model::kMaxStateSets] = {};
   long long *zero_lens_ = nullptr;
   long long zero_max_ = 0;
   void clear_graphs();
@@ -94,7 +94,7 @@ class Batch {
   bool graph_mode_ = false;
   bool persistent_ = false;
   std::vector<hipGraphExec_t> graphs_;  // per shot
-  hipGraphExec_t graph_all_ = nullptr;  // forward_all's loop of both shots
+  hipGraphExec_t graph_all_ = nullptr;  // forward_all's loop of all shots
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double loop_ms_ = 0.0;
   // gradient descriptors [model][shot][parity] with snapshot-store bases

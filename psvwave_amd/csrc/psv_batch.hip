@@ -245,12 +245,13 @@ void Batch::forward(int i_shot) {
 }
 
 bool Batch::lockstep_available() {
-  if (n_shots_ != 2 || !step_is_lean(variant_) || persistent_ ||
+  if (n_shots_ < 2 || n_shots_ > Model::kMaxStateSets || !step_is_lean(variant_) ||
+      persistent_ ||
       no_lockstep_room_ || (long long)ntiles_ * n_shots_ > std::numeric_limits<int>::max())
     return false;
   if (ldescs_) return true;
   try {
-    for (auto &m : models_) m->ensure_second_state();
+    for (auto &m : models_) m->ensure_state_sets(n_shots_);
   } catch (const RuntimeError &) {
     // no room: keep what was allocated and run shot by shot; clear the
     // failed allocation's error so that the next launch check does not
@@ -259,7 +260,7 @@ bool Batch::lockstep_available() {
     no_lockstep_room_ = true;
     return false;
   }
-  ensure_zero_table(1);
+  for (int set = 1; set < n_shots_; ++set) ensure_zero_table(set);
   std::vector<StepArgs> descs;
   for (auto &m : models_)
     for (int s = 0; s < n_shots_; ++s)
@@ -271,8 +272,9 @@ bool Batch::lockstep_available() {
   return true;
 }
 
-// Zero both state sets of every member, then advance `nsteps` steps of
-// every shot with one launch per step (lockstep_available must hold).
+// Zero the state sets of every member (one per shot), then advance `nsteps`
+// steps of every shot with one launch per step (lockstep_available must
+// hold).
 void Batch::enqueue_lockstep_loop(int nsteps) {
   for (int set = 0; set < n_shots_; ++set) enqueue_zero_states(set);
   for (int k = 0; k < nsteps; ++k) {
@@ -410,7 +412,7 @@ double Batch::autotune(int steps, int *best, double *ms_per_variant) {
     if (ms_per_variant) ms_per_variant[k] = -1.0;
     if (step_is_lean(candidates[k]) && !lean) continue;
     set_variant(candidates[k], keep_remap);
-    // the loop forward_all runs with this variant: both shots in lockstep
+    // the loop forward_all runs with this variant: the shots in lockstep
     // where it can, else shot 0 alone; timed per step of one shot
     const bool lock = lockstep_available();
     const int n = std::min(steps, nt_);

@@ -115,21 +115,25 @@ class Model {
   // Pieces of a forward time loop for batched launches across models
   // (psv_batch.hip): the arguments of a launch that reads state buffer set
   // `parity`, zeroing the state, and the receiver scan of two-step loops.
-  // state_set 1: the second state buffer set (ensure_second_state), so that
-  // two shots can advance side by side.
-  StepArgs forward_launch_args(int i_shot, int parity, int state_set = 0) const;
+  // set s > 0: another state buffer set (ensure_state_sets), so that up to
+  // kMaxStateSets shots can advance side by side.
+  StepArgs forward_launch_args(int i_shot, int parity, int set = 0) const;
   // Batched gradients: the arguments of an adjoint launch reading state set
   // `parity` (adjoint sources of shot i_shot, kernel correlation), and a
   // snapshot store for one shot at a time ([slot][iz][ix] per field) whose
   // bases both launch kinds get in StepArgs::snap.
   StepArgs adjoint_launch_args(int i_shot, int parity) const;
   double *const *shot_snapshot_store();
-  void enqueue_zero_state(hipStream_t s, int state_set = 0) const;
+  void enqueue_zero_state(hipStream_t s, int set = 0) const;
   // The ten padded arrays of a state buffer set (2 parities x 5 fields) and
   // their length, for batched zeroing.
-  void state_arrays(int state_set, std::vector<double *> &out) const;
+  void state_arrays(int set, std::vector<double *> &out) const;
   long long padded_points() const { return npad_; }
   void ensure_second_state();
+  // state buffer sets 0 .. n-1 allocated (n <= kMaxStateSets); set 1 is the
+  // second lane's
+  void ensure_state_sets(int n);
+  static constexpr int kMaxStateSets = 8;
   void enqueue_receiver_scan(int i_shot, hipStream_t s) const;
   int tile_count() const { return ntiles_; }
   bool lean_ok() const { return lean_ok_; }
@@ -210,6 +214,8 @@ class Model {
   void run_model_two_lanes(bool simulate_adjoint);
   hipStream_t lane_stream_ = nullptr;
   double *lane_state_[2][5] = {};
+  double *extra_state_[kMaxStateSets - 2][2][5] = {};  // sets 2 ..
+  double *const (*state_set(int set) const)[5];
   std::vector<hipEvent_t> lane_events_;
   double *defer_ = nullptr;  // deferred correlation terms [slot][3][ngrid]
   long long defer_n_ = 0;

@@ -965,10 +965,29 @@ void Model::timed_loop(const std::function<void()> &enqueue) {
   timed_ += launches_ - before;
 }
 
-StepArgs Model::forward_launch_args(int i_shot, int parity, int state_set) const {
+// State buffer set `set`: 0 the model's, 1 the second lane's, 2.. extra.
+double *const (*Model::state_set(int set) const)[5] {
+  if (set < 0 || set >= kMaxStateSets) throw ValueError("state buffer set out of range");
+  return set == 0 ? state_ : set == 1 ? lane_state_ : extra_state_[set - 2];
+}
+
+void Model::ensure_state_sets(int n) {
+  if (n > kMaxStateSets) throw ValueError("too many state buffer sets");
+  if (n > 1) ensure_second_state();
+  bool added = false;
+  for (int k = 2; k < n; ++k)
+    if (!extra_state_[k - 2][0][0]) {
+      for (int p = 0; p < 2; ++p)
+        for (int f = 0; f < 5; ++f) extra_state_[k - 2][p][f] = dalloc(npad_);
+      added = true;
+    }
+  if (added) sync();
+}
+
+StepArgs Model::forward_launch_args(int i_shot, int parity, int set) const {
   check_shot(i_shot);
-  double *const(*state)[5] = state_set ? lane_state_ : state_;
-  if (!state[0][0]) throw RuntimeError("the second state buffer set is not allocated");
+  double *const(*state)[5] = state_set(set);
+  if (!state[0][0]) throw RuntimeError("the state buffer set is not allocated");
   StepArgs a = base_args();
   a.adjoint = 0;
   a.dts = cfg_.dt;
@@ -1026,15 +1045,15 @@ double *const *Model::shot_snapshot_store() {
   return shot_snap_;
 }
 
-void Model::enqueue_zero_state(hipStream_t s, int state_set) const {
-  double *const(*state)[5] = state_set ? lane_state_ : state_;
+void Model::enqueue_zero_state(hipStream_t s, int set) const {
+  double *const(*state)[5] = state_set(set);
   for (int k = 0; k < 2; ++k)
     for (int f = 0; f < 5; ++f)
       hip_check(launch_fill(state[k][f], npad_, 0.0, s), "zero state");
 }
 
-void Model::state_arrays(int state_set, std::vector<double *> &out) const {
-  double *const(*state)[5] = state_set ? lane_state_ : state_;
+void Model::state_arrays(int set, std::vector<double *> &out) const {
+  double *const(*state)[5] = state_set(set);
   for (int k = 0; k < 2; ++k)
     for (int f = 0; f < 5; ++f) out.push_back(state[k][f]);
 }

@@ -194,3 +194,55 @@ def test_batch_gradients_equal_single_models(oracle_lib, variant):
     for a, b in zip(batch.get_kernels(3), o.get_kernels()):
         assert np.array_equal(a, b)
     assert abs(misfits[3] - o.misfit) <= 1e-12 * o.misfit
+
+
+def _members3():
+    """Three members with three shots each (one source per shot)."""
+    cfgs = []
+    for k, (nxi, nzi, npb) in enumerate([(90, 50, 12), (130, 21, 7), (64, 16, 4)]):
+        c = small_config(**{"domain.nt": 97, "domain.nx_inner": nxi,
+                            "domain.nz_inner": nzi, "boundary.np_boundary": npb,
+                            "domain.nx_inner_boundary": 1,
+                            "domain.nz_inner_boundary": 1})
+        c["sources"].update(n_sources=3, n_shots=3, moment_angles=[20 + 30 * k, 200, 110],
+                            ix_sources=[max(1, nxi // 3), max(1, nxi // 2),
+                                        max(1, 2 * nxi // 3)],
+                            iz_sources=[max(1, nzi // 3), max(1, nzi // 2),
+                                        max(1, nzi // 4)],
+                            which_source_to_fire_in_which_shot=[[0], [1], [2]])
+        c["receivers"].update(nr=3, ix_receivers=[0, nxi // 2, nxi - 1],
+                              iz_receivers=[nzi - 1, nzi // 2, 0])
+        cfgs.append(c)
+    return cfgs
+
+
+@pytest.mark.parametrize("variant", [27, 28, 9])
+def test_forward_all_lockstep_three_shots(oracle_lib, variant):
+    """Three shots advance in the same launches, shot s in state buffer set
+    s: the seismograms equal forward(0), forward(1), forward(2) bit for bit,
+    and one member equals the oracle."""
+    import psvwave_amd
+    cfgs = _members3()
+    batch = psvwave_amd.Batch(cfgs)
+    batch.set_variant(variant)
+    fields = []
+    for m in range(len(cfgs)):
+        info = batch.infos[m]
+        fields.append(_fields(info["nx"], info["nz"], 70 + m))
+        batch.set_parameter_fields(m, *fields[m])
+    print(f"[lockstep3] variant {variant}", flush=True)
+    for s in range(3):
+        batch.forward(s)
+    want = [batch.get_synthetic_data(m) for m in range(len(cfgs))]
+    assert batch.lockstep()
+    for rep in range(2):
+        batch.forward_all()
+        for m in range(len(cfgs)):
+            for a, b in zip(batch.get_synthetic_data(m), want[m]):
+                assert np.array_equal(a, b), (variant, m, rep)
+    o = oracle_lib.OracleModel(cfgs[1])
+    o.set_parameter_fields(*fields[1])
+    for s in range(3):
+        o.forward_simulate(s, store_fields=False)
+    for a, b in zip(batch.get_synthetic_data(1), o.get_synthetic_data()):
+        assert np.array_equal(a, b)
